@@ -1,0 +1,261 @@
+/*
+ * sop.h -- C ABI of libsop, the MI355X (gfx950) SOP overdamped dynamics engine.
+ *
+ * Plain C types only: pointers, sizes, status codes.  One engine owns one HIP
+ * device, its memory pools and streams; systems (replicas) live on the device
+ * until removed.  An engine is driven by one host thread and is not
+ * thread-safe, except that the serve loop runs an internal thread behind
+ * sop_serve_submit / sop_serve_poll.
+ *
+ * Reference interface replaced (samchogroup/SOP):
+ *   force/energy entry points  energy.h:13-24 (set_potential, clear_forces,
+ *                              energy_eval, force_eval, random_force,
+ *                              fene_energy, soft_sphere_angular_energy,
+ *                              vdw_energy)
+ *   list entry points          sop.h:32-35 (update_neighbor_list,
+ *                              update_pair_list, update_cell_list,
+ *                              update_hybrid_list)
+ *   integrator                 sop.h:28 (overdamped_iteration)
+ *   device state               GPUvars.h:150-153 (alloc_GPU_arrays,
+ *                              alloc_cudpp, setup_rng)
+ * The reference passes all data through globals (global.h:83-233); here every
+ * entry point takes the engine and a system id instead, and returns a status
+ * code instead of calling exit(-1).
+ */
+#ifndef SOP_H_
+#define SOP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t sop_status;
+
+#define SOP_OK 0
+#define SOP_ERR_INVALID (-1)   /* bad argument / topology */
+#define SOP_ERR_NO_DEVICE (-2) /* no HIP device (the engine has no CPU fallback) */
+#define SOP_ERR_HIP (-3)       /* HIP runtime error */
+#define SOP_ERR_CAPACITY (-4)  /* list / cell capacity overflow or pool full */
+#define SOP_ERR_NUMERIC (-5)   /* non-finite coordinates or overstretched bond */
+#define SOP_ERR_STATE (-6)     /* call not valid in the current state */
+#define SOP_ERR_IO (-7)        /* file read / write failure */
+#define SOP_ERR_BUSY (-8)      /* serve: job not finished / queue full */
+#define SOP_ERR_PBC (-9)       /* system extent too close to half the periodic box */
+
+/* energy slots returned by sop_energy_eval (FP64 sums of FP32 terms) */
+#define SOP_E_BOND 0   /* e_bnd      (energy.cu:194-225) */
+#define SOP_E_ANGLE 1  /* e_ang_ss   (energy.cu:278-307) */
+#define SOP_E_NATIVE 2 /* e_vdw_rr_att (energy.cu:381-410) */
+#define SOP_E_REP 3    /* e_vdw_rr_rep (energy.cu:412-438) */
+#define SOP_E_TOTAL 4  /* rna_etot = sum of the above (energy.cu:76) */
+#define SOP_E_VIRIAL 5 /* sum over pairs of r_ij . f_ij (no reference counterpart) */
+#define SOP_E_COUNT 6
+
+/* lists for sop_get_list: half lists, pairs (i<j) of 0-based bead indices */
+#define SOP_LIST_NL_NATIVE 0 /* idx_neighbor_list_att (global.h:165) */
+#define SOP_LIST_NL_REP 1    /* idx_neighbor_list_rep (global.h:167) */
+#define SOP_LIST_PL_NATIVE 2 /* idx_pair_list_att     (global.h:170) */
+#define SOP_LIST_PL_REP 3    /* idx_pair_list_rep     (global.h:172) */
+
+typedef struct sop_engine sop_engine;
+
+/* Force field and run parameters of one system.  Bead types are 1 or 2 as in
+ * the reference's 3x3 tables (global.cpp:51-92); row/column 0 is unused. */
+typedef struct sop_params {
+  float k_bnd;    /* FENE spring constant, kcal/mol/A^2 (param.cpp:203) */
+  float R0;       /* FENE max deviation, A (param.cpp:204) */
+  float eps_ss;   /* i,i+2 soft-sphere epsilon (param.cpp:226) */
+  float sigma_ss; /* i,i+2 soft-sphere sigma (param.cpp:224) */
+  float eps_att[3][3];   /* native LJ well depth, one table for energy and force */
+  float eps_rep[3][3];   /* non-native repulsion epsilon (global.cpp:57-61) */
+  float sigma_rep[3][3]; /* non-native sigma (global.cpp:75-79) */
+  float rep_c12;  /* repulsion E = eps*(c12*(s/r)^12 + c6*(s/r)^6); reference (1,1) */
+  float rep_c6;
+  /* Cutoffs: factor > 0 gives rc = (float)(factor * (double)base) with base = r0
+   * for natives and sigma_rep for non-natives (sop.cu:746,815,861,910);
+   * factor <= 0 gives rc = abs (absolute cutoff in A). */
+  double nl_factor;
+  float nl_abs;
+  double pl_factor;
+  float pl_abs;
+  int32_t n_nl;   /* neighbor-list rebuild interval, steps (nnlup, param.cpp:292) */
+  int32_t n_pl;   /* pair-list rebuild interval, steps (reference: every step) */
+  float T;        /* temperature, kcal/mol (param.cpp:281) */
+  float zeta;     /* friction, 1/tau_L (param.cpp:289) */
+  float h;        /* time step, tau_L (param.cpp:293) */
+  float boxl;     /* periodic box edge, A; <= 0 disables the minimum image */
+  uint64_t seed;  /* Philox4x32-10 key of this system's random force stream */
+} sop_params;
+
+/* One system: linear chains of beads.  Beads of a chain are contiguous and
+ * consecutive beads of a chain are bonded (reference bonds file, io.cpp:126-148,
+ * with ibead = i, jbead = i+1); the i,i+2 soft-sphere term acts inside a chain
+ * (angles file, io.cpp:149-173).  Non-native repulsion acts on every pair that
+ * is not native and not (same chain and |i-j| <= 2): the reference's master
+ * non-native list (io.cpp:190-226).  Indices are 0-based. */
+typedef struct sop_system_desc {
+  int32_t n_beads;
+  const float *xyz;       /* 3*n unwrapped coordinates, A */
+  const int32_t *chain;   /* n chain ids, non-decreasing; NULL = one chain */
+  const int32_t *type;    /* n bead types in {1,2}; NULL = all 1 */
+  const float *bond_r0;   /* n: equilibrium length of bond (i,i+1); NULL = 3.8 */
+  int32_t n_native;
+  const int32_t *nat_i;   /* native pairs, nat_i[k] < nat_j[k] */
+  const int32_t *nat_j;
+  const float *nat_r0;    /* native distance (lj_nat_pdb_dist, io.cpp:208) */
+  sop_params params;
+  int64_t step0;          /* initial value of the step counter */
+} sop_system_desc;
+
+typedef struct sop_engine_opts {
+  int32_t nl_cap_max;     /* per-bead neighbor-list capacity bound (default 4096) */
+  int32_t pl_cap_max;     /* per-bead pair-list capacity bound (default 1024) */
+  int32_t cell_cap_max;   /* per-system cell-grid capacity bound (default 32768) */
+  int32_t lanes_per_bead; /* force-kernel lanes per bead: 4, 8, 16 or 32; 0 = autotune */
+  int32_t grid_blocks;    /* persistent grid size; 0 = 8 per compute unit */
+  int32_t use_graphs;     /* capture step blocks as hipGraphs (default 1) */
+  int64_t pool_bytes;     /* serve: HBM budget for all pools; 0 = grow on demand */
+} sop_engine_opts;
+
+/* --- parameters ---------------------------------------------------------- */
+/* Reference defaults (param.cpp:203-299, global.cpp:51-92; one eps table). */
+sop_status sop_params_reference(sop_params *p);
+/* Defaults of the benchmark configurations: eps_h = eps_l = 1, sigma = 3.8,
+ * sigma_ss = 5.4, absolute cutoffs 32 A / 16 A every 50 / 10 steps, T = 0.6,
+ * dt = 0.05, zeta = 50, no periodic box. */
+sop_status sop_params_config(sop_params *p);
+/* Derived FP32 coefficients exactly as the engine uses them:
+ * a5 = h/zeta (param.cpp:299), var = sqrt(2 T zeta / h) (energy.cu:158). */
+sop_status sop_params_derived(const sop_params *p, float *a5, float *var);
+
+/* --- engine -------------------------------------------------------------- */
+void sop_engine_opts_default(sop_engine_opts *o);
+/* device >= 0: HIP device ordinal.  Fails with SOP_ERR_NO_DEVICE without one. */
+sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_engine **out);
+void sop_engine_destroy(sop_engine *e);
+const char *sop_last_error(const sop_engine *e);
+const char *sop_version(void);
+sop_status sop_synchronize(sop_engine *e);
+
+/* --- systems (alloc_GPU_arrays / load, GPUvars.cu:99-228, io.cpp:112-259) -- */
+sop_status sop_system_add(sop_engine *e, const sop_system_desc *d, int32_t *sys);
+sop_status sop_system_remove(sop_engine *e, int32_t sys);
+sop_status sop_system_info(sop_engine *e, int32_t sys, int32_t *n_beads, int64_t *step);
+
+/* --- lists (update_neighbor_list / update_pair_list, sop.cu:928-1823) ----- */
+/* Rebuild the neighbor list with the cell hash (counting sort + 27-cell
+ * stencil + wave-level compaction); also covers update_cell_list and
+ * update_hybrid_list, whose intent (a spatial candidate filter) it replaces. */
+sop_status sop_update_neighbor_list(sop_engine *e, int32_t sys);
+sop_status sop_update_pair_list(sop_engine *e, int32_t sys);
+/* Copy a half list (i<j pairs, 2*n int32 as i0,j0,i1,j1,...) to ij; n is the
+ * total count even when it exceeds cap. */
+sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, int64_t cap,
+                        int64_t *n);
+
+/* --- forces and energies (energy.h:13-24) -------------------------------- */
+/* force_eval(): forces at the current coordinates over the current pair
+ * list.  with_noise = 0: deterministic forces only; 1: plus the random force
+ * of the next step (random_force, energy.cu:156-167). */
+sop_status sop_force_eval(sop_engine *e, int32_t sys, int32_t with_noise);
+sop_status sop_get_forces(sop_engine *e, int32_t sys, float *f3n);
+/* energy_eval(): e[SOP_E_COUNT] over the current pair list. */
+sop_status sop_energy_eval(sop_engine *e, int32_t sys, double *energies);
+/* calculate_observables() (sop.cu:528-611): Q, native contact count, Rg^2. */
+sop_status sop_observables(sop_engine *e, int32_t sys, double *Q, int32_t *contacts,
+                           double *rgsq);
+
+/* --- dynamics (overdamped_ctrl / overdamped_iteration, sop.cu:287-520) ----- */
+/* Advance the listed systems by nsteps each, in one batch (one launch per
+ * phase for all systems; hipGraph-captured blocks when aligned). */
+sop_status sop_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps);
+/* Same dynamics, one system per launch, no graphs: the plain per-system path. */
+sop_status sop_run_plain(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps);
+sop_status sop_get_coords(sop_engine *e, int32_t sys, float *xyz, int64_t *step);
+sop_status sop_set_coords(sop_engine *e, int32_t sys, const float *xyz);
+/* Wrapped coordinates pos = unc - boxl*rint(unc/boxl) (sop.cu:510-512). */
+sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz);
+
+/* --- measurement --------------------------------------------------------- */
+typedef struct sop_list_stats {
+  int64_t nl_native, nl_rep, pl_native, pl_rep; /* half-list entry counts */
+  int32_t nl_max_per_bead, pl_max_per_bead;
+} sop_list_stats;
+sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *st);
+/* Time `reps` launches of the fused force+integrate kernel over the listed
+ * systems with HIP events on the engine stream; avg_ms per launch. */
+sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t reps,
+                                double *avg_ms);
+/* Pick lanes-per-bead and grid size for the listed systems by timing. */
+sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes,
+                        int32_t *grid);
+/* Device self test of the wave-level helpers (DPP scan and group sums,
+ * ds_swizzle, ballot prefix): bad64[lane] != 0 flags a mismatch; normals192
+ * receives 64 x 3 device normals for key 0x9abcdef012345678, bead = lane,
+ * step = 1000 + lane. */
+sop_status sop_selftest(sop_engine *e, int32_t *bad64, float *normals192);
+
+/* --- serving: continuous batching ---------------------------------------- */
+typedef struct sop_serve_opts {
+  int64_t hbm_budget_bytes; /* admission budget for job state (0 = 8 GiB) */
+  int32_t max_jobs;         /* resident job slots (0 = 4096) */
+  int32_t block_steps;      /* steps per graph replay (0 = n_nl of the first job) */
+} sop_serve_opts;
+
+typedef struct sop_job_result {
+  int32_t done;          /* 1 when finished and copied back */
+  int32_t status;        /* SOP_OK or the job's error */
+  int32_t n_beads;
+  int64_t steps;
+  double t_submit, t_admit, t_done; /* seconds on the engine clock */
+  double energies[SOP_E_COUNT];
+} sop_job_result;
+
+sop_status sop_serve_start(sop_engine *e, const sop_serve_opts *o);
+/* Copies the description; the job runs nsteps from its step0. */
+sop_status sop_serve_submit(sop_engine *e, const sop_system_desc *d, int64_t nsteps,
+                            int64_t *job);
+/* Non-blocking; when done, xyz (3n floats, may be NULL) receives the final
+ * unwrapped coordinates. SOP_ERR_BUSY while the job is queued or running. */
+sop_status sop_serve_poll(sop_engine *e, int64_t job, sop_job_result *r, float *xyz);
+sop_status sop_serve_stop(sop_engine *e);
+double sop_clock_seconds(void);
+
+/* --- host-only utilities (no device needed) ------------------------------ */
+typedef struct sop_gen_opts {
+  float bond;         /* walk step, A (3.8) */
+  float min_dist;     /* reject beads closer than this to a non-bonded bead (4.0) */
+  float volume_per_bead; /* compact target: confining sphere of n*v volume (140 A^3) */
+  int32_t relax_steps;   /* steepest-descent steps after the walk (0) */
+} sop_gen_opts;
+void sop_gen_opts_default(sop_gen_opts *o);
+/* Seeded compact self-avoiding walk: xyz (3n), bond_r0 (n; r0 of (i,i+1)). */
+sop_status sop_generate_chain(uint64_t seed, int32_t n, const sop_gen_opts *o, float *xyz,
+                              float *bond_r0);
+/* Native map: pairs with |i-j| > 2 in a chain (and, unless intra_only, pairs
+ * on different chains) closer than cutoff; r0 = their distance.  Pairs come
+ * out sorted (i, then j).  Two-call: pass cap = 0 to get *n_nat. */
+sop_status sop_native_contacts(int32_t n, const float *xyz, const int32_t *chain, float cutoff,
+                               int32_t intra_only, int32_t *nat_i, int32_t *nat_j, float *nat_r0,
+                               int64_t cap, int64_t *n_nat);
+
+/* Reference input files (io.cpp:112-259): bonds, angles, vdw, init.  The
+ * topology object owns the arrays; sop_topology_desc points into it. */
+typedef struct sop_topology sop_topology;
+sop_status sop_topology_load(const char *bonds, const char *angles, const char *vdw,
+                             const char *init, sop_topology **out);
+sop_status sop_topology_desc(const sop_topology *t, sop_system_desc *d);
+void sop_topology_free(sop_topology *t);
+/* Writers in the reference formats: record_traj appends "%f %f %f" lines
+ * (io.cpp:261-282); save_coords writes "%d %f %f %f" (io.cpp:284-305). */
+sop_status sop_write_traj(const char *path, int32_t n, const float *xyz);
+sop_status sop_save_coords(const char *path, int32_t n, const float *xyz);
+sop_status sop_load_coords(const char *path, int32_t n, float *xyz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOP_H_ */

@@ -1,0 +1,1094 @@
+// engine.cpp -- libsop host engine: device pools, systems, batch work tables,
+// hipGraph step blocks and the C ABI of include/sop.h (device part).
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+using namespace sopk;
+
+namespace sopk {
+
+// ---------------------------------------------------------------- allocator
+
+void RangeAlloc::init(int64_t cap) {
+  cap_ = cap;
+  used_ = 0;
+  free_.clear();
+  if (cap > 0) free_[0] = cap;
+}
+
+int64_t RangeAlloc::alloc(int64_t n, int64_t align) {
+  if (n <= 0) n = 1;
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    const int64_t off = it->first, len = it->second;
+    const int64_t a = (off + align - 1) / align * align;
+    if (a + n > off + len) continue;
+    free_.erase(it);
+    if (a > off) free_[off] = a - off;
+    if (a + n < off + len) free_[a + n] = off + len - (a + n);
+    used_ += n;
+    return a;
+  }
+  return -1;
+}
+
+void RangeAlloc::release(int64_t off, int64_t n) {
+  if (n <= 0) n = 1;
+  used_ -= n;
+  auto it = free_.emplace(off, n).first;
+  auto nx = std::next(it);
+  if (nx != free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    free_.erase(nx);
+  }
+  if (it != free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+void RangeAlloc::grow(int64_t new_cap) {
+  if (new_cap <= cap_) return;
+  release(cap_, new_cap - cap_);
+  used_ += new_cap - cap_;  // release() subtracted it
+  cap_ = new_cap;
+}
+
+sop_status set_error(sop_engine *e, sop_status st, const std::string &msg) {
+  if (e) e->err = msg;
+  return st;
+}
+
+sop_status hip_check(sop_engine *e, hipError_t h, const char *what) {
+  if (h == hipSuccess) return SOP_OK;
+  return set_error(e, SOP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(h));
+}
+
+int grid_tiles(const sop_engine *e) {
+  int g = e->opts.grid_blocks > 0 ? e->opts.grid_blocks : 8 * e->num_cu;
+  g = std::max(8, g / 8 * 8);
+  const int need = (e->cur_nwork + 7) / 8 * 8;
+  return std::max(8, std::min(g, need));
+}
+
+int grid_sys(const sop_engine *e) { return std::max(1, std::min(e->cur_nsys, 2048)); }
+
+} // namespace sopk
+
+#define HIPCK(expr)                                            \
+  do {                                                         \
+    sop_status _s = hip_check(e, (expr), #expr);               \
+    if (_s != SOP_OK) return _s;                               \
+  } while (0)
+
+// ----------------------------------------------------------------- pools
+
+static size_t bead_elem_bytes[] = {16, 16, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 16, 48};
+
+static void bind_arrays(sop_engine *e) {
+  DevPtrs &p = e->p;
+  e->bead_arrays = {
+      {(void **)&p.pos0, 16},      {(void **)&p.pos1, 16},       {(void **)&p.bond_r0, 4},
+      {(void **)&p.nat_start, 4},  {(void **)&p.nat_cnt, 4},     {(void **)&p.nlatt_cnt, 4},
+      {(void **)&p.platt_cnt, 4},  {(void **)&p.nlrep_cnt, 4},   {(void **)&p.plrep_cnt, 4},
+      {(void **)&p.cell_of, 4},    {(void **)&p.cell_tmp, 4},    {(void **)&p.cell_sorted, 4},
+      {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}};
+  e->nat_arrays = {{(void **)&p.nat_j, 4},   {(void **)&p.nat_r0, 4},   {(void **)&p.nlatt_j, 4},
+                   {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
+  e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4}};
+  (void)bead_elem_bytes;
+}
+
+static sop_status grow_array(sop_engine *e, void **ptr, size_t old_bytes, size_t new_bytes) {
+  void *np = nullptr;
+  HIPCK(hipMalloc(&np, std::max<size_t>(new_bytes, 256)));
+  if (*ptr && old_bytes) HIPCK(hipMemcpy(np, *ptr, old_bytes, hipMemcpyDeviceToDevice));
+  if (*ptr) HIPCK(hipFree(*ptr));
+  *ptr = np;
+  return SOP_OK;
+}
+
+static void invalidate_graphs(sop_engine *e) {
+  for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+  e->graphs.clear();
+}
+
+// Grow pools so that the requested extra capacity fits (doubling).
+static sop_status ensure_capacity(sop_engine *e, int64_t beads, int64_t nat, int64_t nl, int64_t pl,
+                                  int64_t cells, int64_t slots) {
+  Pools &c = e->cap;
+  bool changed = false;
+  auto next = [](int64_t cur, int64_t need) {
+    int64_t v = std::max<int64_t>(cur, 1);
+    while (v < need) v *= 2;
+    return v;
+  };
+  HIPCK(hipStreamSynchronize(e->stream));
+  if (beads > c.beads) {
+    const int64_t nb = next(c.beads, beads);
+    for (auto &a : e->bead_arrays) {
+      sop_status s = grow_array(e, a.first, a.second * (size_t)c.beads, a.second * (size_t)nb);
+      if (s) return s;
+    }
+    sop_status s = grow_array(e, (void **)&e->p.tile_sys, 4 * (size_t)(c.beads / TILE), 4 * (size_t)(nb / TILE));
+    if (s) return s;
+    e->a_beads.grow(nb);
+    c.beads = nb;
+    changed = true;
+  }
+  if (nat > c.nat) {
+    const int64_t nn = next(c.nat, nat);
+    for (auto &a : e->nat_arrays) {
+      sop_status s = grow_array(e, a.first, a.second * (size_t)c.nat, a.second * (size_t)nn);
+      if (s) return s;
+    }
+    e->a_nat.grow(nn);
+    c.nat = nn;
+    changed = true;
+  }
+  if (nl > c.nl) {
+    const int64_t nn = next(c.nl, nl);
+    sop_status s = grow_array(e, (void **)&e->p.nlrep_j, 4 * (size_t)c.nl, 4 * (size_t)nn);
+    if (s) return s;
+    e->a_nl.grow(nn);
+    c.nl = nn;
+    changed = true;
+  }
+  if (pl > c.pl) {
+    const int64_t nn = next(c.pl, pl);
+    sop_status s = grow_array(e, (void **)&e->p.plrep_j, 4 * (size_t)c.pl, 4 * (size_t)nn);
+    if (s) return s;
+    e->a_pl.grow(nn);
+    c.pl = nn;
+    changed = true;
+  }
+  if (cells > c.cells) {
+    const int64_t nn = next(c.cells, cells);
+    for (auto &a : e->cell_arrays) {
+      sop_status s = grow_array(e, a.first, a.second * (size_t)c.cells, a.second * (size_t)nn);
+      if (s) return s;
+    }
+    e->a_cells.grow(nn);
+    c.cells = nn;
+    changed = true;
+  }
+  if (slots > c.slots) {
+    const int64_t nn = next(c.slots, slots);
+    sop_status s = grow_array(e, (void **)&e->p.sys, sizeof(SysDev) * (size_t)c.slots, sizeof(SysDev) * (size_t)nn);
+    if (s) return s;
+    s = grow_array(e, (void **)&e->p.sysout, 8 * SYSOUT_N * (size_t)c.slots, 8 * SYSOUT_N * (size_t)nn);
+    if (s) return s;
+    s = grow_array(e, (void **)&e->p.syslist, 4 * (size_t)c.slots, 4 * (size_t)nn);
+    if (s) return s;
+    c.slots = nn;
+    e->sys.resize((size_t)nn);
+    changed = true;
+  }
+  if (changed) invalidate_graphs(e);
+  return SOP_OK;
+}
+
+// Pool growth on demand: try to allocate, grow and retry once.
+static int64_t alloc_or_grow(sop_engine *e, RangeAlloc &a, int64_t n, int64_t align, int which,
+                             sop_status *st) {
+  *st = SOP_OK;
+  int64_t off = a.alloc(n, align);
+  if (off >= 0) return off;
+  if (e->fixed_pools) {
+    *st = set_error(e, SOP_ERR_CAPACITY, "memory pool full");
+    return -1;
+  }
+  const int64_t need = a.cap() + n + align;
+  int64_t req[5] = {0, 0, 0, 0, 0};
+  req[which] = need;
+  *st = ensure_capacity(e, req[0], req[1], req[2], req[3], req[4], 0);
+  if (*st) return -1;
+  off = a.alloc(n, align);
+  if (off < 0) *st = set_error(e, SOP_ERR_CAPACITY, "pool allocation failed after growth");
+  return off;
+}
+
+// --------------------------------------------------------------- helpers
+
+static float cutoff_host(double factor, float absval, float base) {
+  return factor > 0.0 ? (float)(factor * (double)base) : absval;
+}
+
+static inline int32_t round_up(int32_t v, int32_t a) { return (v + a - 1) / a * a; }
+
+namespace sopk {
+
+sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out) {
+  HIPCK(hipMemcpyAsync(out, e->p.sys + slot, sizeof(SysDev), hipMemcpyDeviceToHost, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return SOP_OK;
+}
+
+int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
+  const int64_t n = d->n_beads;
+  const int64_t nl_cap = round_up((int32_t)std::max<int64_t>(1, std::min<int64_t>(n - 1, e->opts.nl_cap_max)), 16);
+  const int64_t pl_cap = round_up((int32_t)std::max<int64_t>(1, std::min<int64_t>(n - 1, e->opts.pl_cap_max)), 16);
+  const int64_t beads = (n + TILE - 1) / TILE * TILE;
+  const int64_t cell_cap = std::min<int64_t>(e->opts.cell_cap_max, std::max<int64_t>(64, 2 * n)) + 1;
+  return beads * 152 + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + pl_cap) + 12 * cell_cap;
+}
+
+// Validate and upload one system; builds its initial neighbor and pair lists.
+sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *slot_out) {
+  if (!d || d->n_beads < 1 || !d->xyz) return set_error(e, SOP_ERR_INVALID, "system: need n_beads >= 1 and xyz");
+  const int32_t n = d->n_beads;
+  const sop_params &P = d->params;
+  if (P.n_nl < 1 || P.n_pl < 1) return set_error(e, SOP_ERR_INVALID, "n_nl and n_pl must be >= 1");
+  if (!(P.zeta > 0.0f) || !(P.h > 0.0f) || !(P.T >= 0.0f) || !(P.R0 > 0.0f))
+    return set_error(e, SOP_ERR_INVALID, "need zeta > 0, h > 0, T >= 0, R0 > 0");
+  if (!(P.nl_factor > 0.0) && !(P.nl_abs > 0.0f)) return set_error(e, SOP_ERR_INVALID, "neighbor-list cutoff");
+  if (!(P.pl_factor > 0.0) && !(P.pl_abs > 0.0f)) return set_error(e, SOP_ERR_INVALID, "pair-list cutoff");
+  // chains and types
+  std::vector<int32_t> chain(n, 0), type(n, 1);
+  for (int i = 0; i < n; ++i) {
+    if (d->chain) chain[i] = d->chain[i];
+    if (d->type) type[i] = d->type[i];
+    if (type[i] != 1 && type[i] != 2) return set_error(e, SOP_ERR_INVALID, "bead types must be 1 or 2");
+    if (chain[i] < 0 || chain[i] >= (1 << 28)) return set_error(e, SOP_ERR_INVALID, "chain id out of range");
+    if (i > 0 && chain[i] < chain[i - 1]) return set_error(e, SOP_ERR_INVALID, "chain ids must be non-decreasing");
+  }
+  // natives: validate, two-sided CSR sorted by partner
+  const int32_t nn = d->n_native;
+  if (nn < 0 || (nn > 0 && (!d->nat_i || !d->nat_j || !d->nat_r0)))
+    return set_error(e, SOP_ERR_INVALID, "native arrays");
+  std::vector<std::vector<std::pair<int32_t, float>>> nat(n);
+  for (int k = 0; k < nn; ++k) {
+    const int i = d->nat_i[k], j = d->nat_j[k];
+    if (i < 0 || j < 0 || i >= n || j >= n || i >= j)
+      return set_error(e, SOP_ERR_INVALID, "native pair must satisfy 0 <= i < j < n");
+    if (chain[i] == chain[j] && j - i <= 2)
+      return set_error(e, SOP_ERR_INVALID, "native pair inside the |i-j| <= 2 exclusion");
+    if (!(d->nat_r0[k] > 0.0f)) return set_error(e, SOP_ERR_INVALID, "native r0 must be > 0");
+    nat[i].push_back({j, d->nat_r0[k]});
+    nat[j].push_back({i, d->nat_r0[k]});
+  }
+  for (int i = 0; i < n; ++i) {
+    std::sort(nat[i].begin(), nat[i].end());
+    for (size_t q = 1; q < nat[i].size(); ++q)
+      if (nat[i][q].first == nat[i][q - 1].first) return set_error(e, SOP_ERR_INVALID, "duplicate native pair");
+  }
+  // capacities
+  const int32_t nl_cap = round_up(std::max(1, std::min(n - 1, e->opts.nl_cap_max)), 16);
+  const int32_t pl_cap = round_up(std::max(1, std::min(n - 1, e->opts.pl_cap_max)), 16);
+  const int32_t cell_cap = std::min(e->opts.cell_cap_max, std::max(64, 2 * n));
+  int32_t dmax = 1;
+  while ((int64_t)(dmax + 1) * (dmax + 1) * (dmax + 1) <= cell_cap) ++dmax;
+  SysHost h;
+  h.used = true;
+  h.n = n;
+  h.n_nat = nn;
+  h.bead_len = round_up(n, TILE);
+  h.nat_len = std::max(1, 2 * nn);
+  h.nl_len = (int64_t)n * nl_cap;
+  h.pl_len = (int64_t)n * pl_cap;
+  h.cell_len = cell_cap + 1;
+  h.nl_cap = nl_cap;
+  h.pl_cap = pl_cap;
+  h.cell_cap = cell_cap;
+  h.step = d->step0;
+  h.parity = 0;
+  h.params = P;
+  sop_status st;
+  h.bead_off = alloc_or_grow(e, e->a_beads, h.bead_len, TILE, 0, &st);
+  if (st) return st;
+  h.nat_off = alloc_or_grow(e, e->a_nat, h.nat_len, 1, 1, &st);
+  if (st) { e->a_beads.release(h.bead_off, h.bead_len); return st; }
+  h.nl_off = alloc_or_grow(e, e->a_nl, h.nl_len, 16, 2, &st);
+  if (st) { e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len); return st; }
+  h.pl_off = alloc_or_grow(e, e->a_pl, h.pl_len, 16, 3, &st);
+  if (st) {
+    e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
+    e->a_nl.release(h.nl_off, h.nl_len);
+    return st;
+  }
+  h.cell_off = alloc_or_grow(e, e->a_cells, h.cell_len, 1, 4, &st);
+  if (st) {
+    e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
+    e->a_nl.release(h.nl_off, h.nl_len); e->a_pl.release(h.pl_off, h.pl_len);
+    return st;
+  }
+  // slot
+  int32_t slot = -1;
+  for (size_t s = 0; s < e->sys.size(); ++s)
+    if (!e->sys[s].used) { slot = (int32_t)s; break; }
+  if (slot < 0) {
+    if (e->fixed_pools && (int64_t)e->sys.size() >= e->cap.slots && e->cap.slots > 0) {
+      slot = -1;
+    } else {
+      const int64_t want = e->cap.slots + 1;
+      st = ensure_capacity(e, 0, 0, 0, 0, 0, want);
+      if (st) return st;
+      for (size_t s = 0; s < e->sys.size(); ++s)
+        if (!e->sys[s].used) { slot = (int32_t)s; break; }
+    }
+    if (slot < 0) return set_error(e, SOP_ERR_CAPACITY, "no free system slot");
+  }
+  // host staging
+  const int64_t b0 = h.bead_off;
+  std::vector<float4> pos((size_t)h.bead_len, make_float4(0, 0, 0, 0));
+  std::vector<float> bond((size_t)h.bead_len, 0.0f);
+  std::vector<int32_t> nstart((size_t)h.bead_len, (int32_t)h.nat_off), ncnt((size_t)h.bead_len, 0);
+  std::vector<int32_t> zeros((size_t)h.bead_len, 0);
+  std::vector<int32_t> natj((size_t)h.nat_len, 0);
+  std::vector<float> natr((size_t)h.nat_len, 0.0f);
+  h.wbits.resize((size_t)n);
+  int32_t cur = 0;
+  for (int i = 0; i < n; ++i) {
+    const int32_t wb = (chain[i] << 2) | type[i];
+    h.wbits[i] = wb;
+    float wf;
+    std::memcpy(&wf, &wb, 4);
+    pos[i] = make_float4(d->xyz[3 * i], d->xyz[3 * i + 1], d->xyz[3 * i + 2], wf);
+    bond[i] = d->bond_r0 ? d->bond_r0[i] : 3.8f;
+    nstart[i] = (int32_t)h.nat_off + cur;
+    ncnt[i] = (int32_t)nat[i].size();
+    for (auto &pr : nat[i]) {
+      natj[cur] = (int32_t)b0 + pr.first;
+      natr[cur] = pr.second;
+      ++cur;
+    }
+  }
+  const DevPtrs &p = e->p;
+  HIPCK(hipMemcpy(p.pos0 + b0, pos.data(), 16 * h.bead_len, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(p.pos1 + b0, pos.data(), 16 * h.bead_len, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(p.bond_r0 + b0, bond.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(p.nat_start + b0, nstart.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(p.nat_cnt + b0, ncnt.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
+  for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt})
+    HIPCK(hipMemcpy(arr + b0, zeros.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(p.nat_j + h.nat_off, natj.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(p.nat_r0 + h.nat_off, natr.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
+  std::vector<int32_t> ts((size_t)(h.bead_len / TILE), slot);
+  HIPCK(hipMemcpy(p.tile_sys + b0 / TILE, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice));
+  // device descriptor
+  SysDev S;
+  std::memset(&S, 0, sizeof(S));
+  S.bead_off = (int32_t)b0;
+  S.n = n;
+  S.nl_cap = nl_cap;
+  S.pl_cap = pl_cap;
+  S.nl_base = h.nl_off;
+  S.pl_base = h.pl_off;
+  S.cell_base = (int32_t)h.cell_off;
+  S.cell_cap = cell_cap;
+  S.dmax = dmax;
+  S.parity = 0;
+  S.step = d->step0;
+  S.step_end = d->step0;
+  S.n_nl = P.n_nl;
+  S.n_pl = P.n_pl;
+  float a5, var;
+  sop_params_derived(&P, &a5, &var);
+  S.a5 = a5;
+  S.var = var;
+  S.boxl = P.boxl;
+  S.inv_boxl = P.boxl > 0.0f ? 1.0f / P.boxl : 0.0f;
+  S.seed_lo = (uint32_t)P.seed;
+  S.seed_hi = (uint32_t)(P.seed >> 32);
+  S.k_bnd = P.k_bnd;
+  S.inv_R0sq = 1.0f / (P.R0 * P.R0);
+  S.half_k_R0sq = 0.5f * P.k_bnd * P.R0 * P.R0;
+  const double s6 = std::pow((double)P.sigma_ss, 6.0);
+  S.ss_e6 = (float)((double)P.eps_ss * s6);
+  S.ss_f6 = (float)(6.0 * (double)P.eps_ss * s6);
+  float max_rc = 0.0f;
+  for (int ti = 1; ti <= 2; ++ti)
+    for (int tj = 1; tj <= 2; ++tj) {
+      const int q = (ti - 1) * 2 + (tj - 1);
+      const float sg = P.sigma_rep[ti][tj], ep = P.eps_rep[ti][tj];
+      S.eps_att[q] = P.eps_att[ti][tj];
+      S.eps_rep[q] = ep;
+      S.sig2_rep[q] = sg * sg;
+      S.rep_guard[q] = sg * 1.0e-7f;
+      S.rep_m12[q] = -12.0f * ep * P.rep_c12;
+      S.rep_m6[q] = -6.0f * ep * P.rep_c6;
+      const float rn = cutoff_host(P.nl_factor, P.nl_abs, sg);
+      const float rp = cutoff_host(P.pl_factor, P.pl_abs, sg);
+      S.rc2_nl_rep[q] = rn * rn;
+      S.rc2_pl_rep[q] = rp * rp;
+      max_rc = std::max(max_rc, rn);
+    }
+  S.c12 = P.rep_c12;
+  S.c6 = P.rep_c6;
+  S.nl_factor = P.nl_factor;
+  S.pl_factor = P.pl_factor;
+  S.nl_abs = P.nl_abs;
+  S.pl_abs = P.pl_abs;
+  S.cell_min_edge = std::max(1.0e-3f, max_rc * 1.001f);
+  S.wrap_margin = 10.0f;
+  S.dims[0] = S.dims[1] = S.dims[2] = 1;
+  S.ncells = 1;
+  S.wrap_free = 0;
+  HIPCK(hipMemcpy(p.sys + slot, &S, sizeof(SysDev), hipMemcpyHostToDevice));
+  e->sys[slot] = std::move(h);
+  // initial lists (the reference builds both before its step loop)
+  st = engine_set_batch(e, &slot, 1);
+  if (st) return st;
+  HIPCK(launch_nl(e->p, grid_tiles(e), grid_sys(e), 0, 1, e->stream));
+  HIPCK(launch_pl(e->p, grid_tiles(e), 0, 1, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  st = engine_check_errors(e, &slot, 1);
+  if (st) {
+    engine_remove_system(e, slot);
+    return st;
+  }
+  *slot_out = slot;
+  return SOP_OK;
+}
+
+sop_status engine_remove_system(sop_engine *e, int32_t slot) {
+  if (slot < 0 || slot >= (int32_t)e->sys.size() || !e->sys[slot].used)
+    return set_error(e, SOP_ERR_INVALID, "no such system");
+  SysHost &h = e->sys[slot];
+  e->a_beads.release(h.bead_off, h.bead_len);
+  e->a_nat.release(h.nat_off, h.nat_len);
+  e->a_nl.release(h.nl_off, h.nl_len);
+  e->a_pl.release(h.pl_off, h.pl_len);
+  e->a_cells.release(h.cell_off, h.cell_len);
+  h = SysHost();
+  return SOP_OK;
+}
+
+// Upload the work tables for a batch of systems.  Systems are spread over the
+// 8 XCDs by bead count; entry k of the work list is processed by a block on
+// XCD k % 8 (round-robin dispatch with a grid that is a multiple of 8), so all
+// tiles of a system share one XCD's L2.
+sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots) {
+  constexpr int NX = 8;
+  std::vector<int32_t> order(slots, slots + nslots);
+  std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    if (e->sys[a].bead_len != e->sys[b].bead_len) return e->sys[a].bead_len > e->sys[b].bead_len;
+    return a < b;
+  });
+  std::vector<std::vector<int32_t>> bins(NX);
+  std::vector<int64_t> load(NX, 0);
+  for (int32_t s : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    const SysHost &h = e->sys[s];
+    for (int64_t t = h.bead_off / TILE; t < (h.bead_off + h.n + TILE - 1) / TILE; ++t) bins[x].push_back((int32_t)t);
+    load[x] += (h.n + TILE - 1) / TILE;
+  }
+  size_t mx = 0;
+  for (auto &b : bins) mx = std::max(mx, b.size());
+  e->h_work.assign(mx * NX, -1);
+  for (int x = 0; x < NX; ++x)
+    for (size_t m = 0; m < bins[x].size(); ++m) e->h_work[m * NX + x] = bins[x][m];
+  e->h_syslist.assign(slots, slots + nslots);
+  if ((int64_t)e->h_work.size() + 2 > e->work_cap) {
+    HIPCK(hipStreamSynchronize(e->stream));
+    int64_t nc = std::max<int64_t>(1024, (int64_t)e->h_work.size() * 2 + 2);
+    if (e->p.work) HIPCK(hipFree(e->p.work));
+    HIPCK(hipMalloc((void **)&e->p.work, 4 * (size_t)nc));
+    e->work_cap = nc;
+    invalidate_graphs(e);
+  }
+  if (!e->p.nwork) {
+    HIPCK(hipMalloc((void **)&e->p.nwork, 256));
+    e->p.nsys = e->p.nwork + 16;
+    invalidate_graphs(e);
+  }
+  if (nslots > e->cap.slots) return set_error(e, SOP_ERR_INVALID, "batch larger than slot capacity");
+  const int32_t nw = (int32_t)e->h_work.size(), ns = nslots;
+  if (nw > 0) HIPCK(hipMemcpy(e->p.work, e->h_work.data(), 4 * (size_t)nw, hipMemcpyHostToDevice));
+  if (ns > 0) HIPCK(hipMemcpy(e->p.syslist, e->h_syslist.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
+  e->cur_nwork = nw;
+  e->cur_nsys = ns;
+  return SOP_OK;
+}
+
+sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslots) {
+  std::vector<SysDev> sd((size_t)e->cap.slots);
+  HIPCK(hipMemcpy(sd.data(), e->p.sys, sizeof(SysDev) * sd.size(), hipMemcpyDeviceToHost));
+  for (int q = 0; q < nslots; ++q) {
+    const int s = slots[q];
+    const int err = sd[s].err;
+    if (!err) continue;
+    std::ostringstream os;
+    os << "system " << s << ":";
+    sop_status st = SOP_ERR_STATE;
+    if (err & ERR_NAN) { os << " non-finite coordinates;"; st = SOP_ERR_NUMERIC; }
+    if (err & ERR_BOND) { os << " FENE bond stretched to |r-r0| >= R0;"; st = SOP_ERR_NUMERIC; }
+    if (err & ERR_NL_OVF) { os << " neighbor list over capacity;"; st = SOP_ERR_CAPACITY; }
+    if (err & ERR_PL_OVF) { os << " pair list over capacity;"; st = SOP_ERR_CAPACITY; }
+    if (err & ERR_PBC) { os << " extent within 10 A of boxl/2 (open cell grid needs the minimum image to be the identity);"; st = SOP_ERR_PBC; }
+    if (err & ERR_INTERNAL) { os << " inconsistent cell histogram;"; st = SOP_ERR_STATE; }
+    return set_error(e, st, os.str());
+  }
+  return SOP_OK;
+}
+
+static std::string graph_key(const sop_engine *e, int G, int n_nl, int n_pl) {
+  std::ostringstream os;
+  os << G << '|' << n_nl << '|' << n_pl << '|' << e->opts.lanes_per_bead << '|' << grid_tiles(e) << '|'
+     << grid_sys(e);
+  return os.str();
+}
+
+// One hipGraph replay = G steps of an aligned batch (every system's step is a
+// multiple of n_nl): NL rebuild at k % n_nl == 0, PL filter at k % n_pl == 0,
+// fused step kernel every k, step-counter advance at the end.
+sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
+  const SysHost &h0 = e->sys[e->h_syslist[0]];
+  const int n_nl = h0.params.n_nl, n_pl = h0.params.n_pl;
+  const std::string key = graph_key(e, G, n_nl, n_pl);
+  auto it = e->graphs.find(key);
+  hipGraphExec_t exec;
+  if (it == e->graphs.end()) {
+    hipGraph_t graph;
+    const int gt = grid_tiles(e), gs = grid_sys(e), lpb = e->opts.lanes_per_bead;
+    HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < G; ++k) {
+      if (k % n_nl == 0) (void)launch_nl(e->p, gt, gs, k, 0, e->stream);
+      if (k % n_pl == 0) (void)launch_pl(e->p, gt, k, 0, e->stream);
+      (void)launch_step(e->p, MODE_STEP, lpb, gt, k, e->stream);
+    }
+    (void)launch_advance(e->p, G, e->stream);
+    HIPCK(hipStreamEndCapture(e->stream, &graph));
+    HIPCK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCK(hipGraphDestroy(graph));
+    e->graphs[key] = exec;
+  } else {
+    exec = it->second;
+  }
+  HIPCK(hipGraphLaunch(exec, e->stream));
+  return SOP_OK;
+}
+
+// Host-driven steps for the current batch: launches exactly the rebuilds some
+// system is due for (the kernels check each system's own schedule).
+sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
+  if (nsteps <= 0) return SOP_OK;
+  const int gt = grid_tiles(e), gs = grid_sys(e), lpb = e->opts.lanes_per_bead;
+  std::vector<int64_t> st;
+  std::vector<int32_t> nnl, npl;
+  for (int32_t s : e->h_syslist) {
+    st.push_back(e->sys[s].step);
+    nnl.push_back(e->sys[s].params.n_nl);
+    npl.push_back(e->sys[s].params.n_pl);
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    bool any_nl = false, any_pl = false;
+    for (size_t q = 0; q < st.size(); ++q) {
+      any_nl |= ((st[q] + k) % nnl[q]) == 0;
+      any_pl |= ((st[q] + k) % npl[q]) == 0;
+    }
+    if (any_nl) HIPCK(launch_nl(e->p, gt, gs, (int)k, 0, e->stream));
+    if (any_pl) HIPCK(launch_pl(e->p, gt, (int)k, 0, e->stream));
+    HIPCK(launch_step(e->p, MODE_STEP, lpb, gt, (int)k, e->stream));
+  }
+  HIPCK(launch_advance(e->p, (int)nsteps, e->stream));
+  return SOP_OK;
+}
+
+} // namespace sopk
+
+// ================================================================ C ABI
+
+extern "C" {
+
+const char *sop_version(void) { return "sop-mi355x 0.1 (gfx950)"; }
+
+double sop_clock_seconds(void) {
+  using namespace std::chrono;
+  return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+void sop_engine_opts_default(sop_engine_opts *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->nl_cap_max = 4096;
+  o->pl_cap_max = 1024;
+  o->cell_cap_max = 32768;
+  o->lanes_per_bead = 16;
+  o->grid_blocks = 0;
+  o->use_graphs = 1;
+  o->pool_bytes = 0;
+}
+
+sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_engine **out) {
+  if (!out) return SOP_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SOP_ERR_NO_DEVICE;
+  if (device < 0 || device >= count) return SOP_ERR_NO_DEVICE;
+  sop_engine *e = new sop_engine();
+  sop_engine_opts_default(&e->opts);
+  if (opts) {
+    e->opts = *opts;
+    sop_engine_opts def;
+    sop_engine_opts_default(&def);
+    if (e->opts.nl_cap_max <= 0) e->opts.nl_cap_max = def.nl_cap_max;
+    if (e->opts.pl_cap_max <= 0) e->opts.pl_cap_max = def.pl_cap_max;
+    if (e->opts.cell_cap_max <= 0) e->opts.cell_cap_max = def.cell_cap_max;
+    if (e->opts.lanes_per_bead <= 0) e->opts.lanes_per_bead = def.lanes_per_bead;
+  }
+  const int lpb = e->opts.lanes_per_bead;
+  if (lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) {
+    delete e;
+    return SOP_ERR_INVALID;
+  }
+  e->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete e;
+    return SOP_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete e;
+    return SOP_ERR_HIP;
+  }
+  e->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return SOP_ERR_HIP;
+  }
+  bind_arrays(e);
+  // Lane-layout self test of the DPP / swizzle / ballot helpers.
+  int *d_out = nullptr;
+  float *d_f = nullptr;
+  int h_out[64];
+  bool ok = hipMalloc((void **)&d_out, 64 * 4) == hipSuccess && hipMalloc((void **)&d_f, 64 * 12) == hipSuccess &&
+            launch_selftest(d_out, d_f, e->stream) == hipSuccess &&
+            hipMemcpy(h_out, d_out, 64 * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  for (int k = 0; ok && k < 64; ++k) ok = h_out[k] == 0;
+  (void)hipFree(d_out);
+  (void)hipFree(d_f);
+  if (!ok) {
+    sop_engine_destroy(e);
+    return SOP_ERR_HIP;
+  }
+  if (e->opts.pool_bytes > 0) {
+    // fixed pools sized from the budget (serving): split by typical use
+    // (per bead ~152 B + 4*(nl_cap+pl_cap) B; lists dominate)
+    const int64_t B = e->opts.pool_bytes;
+    const int64_t beads = std::max<int64_t>(4096, B / 8 / 152) / TILE * TILE;
+    const int64_t nat = std::max<int64_t>(4096, B / 64 / 24);
+    const int64_t nl = std::max<int64_t>(65536, B * 5 / 8 / 4);
+    const int64_t pl = std::max<int64_t>(65536, B / 8 / 4);
+    const int64_t cells = std::max<int64_t>(65536, B / 64 / 12);
+    sop_status st = ensure_capacity(e, beads, nat, nl, pl, cells, 4096);
+    if (st) {
+      sop_engine_destroy(e);
+      return st;
+    }
+    e->fixed_pools = true;
+  }
+  *out = e;
+  return SOP_OK;
+}
+
+void sop_engine_destroy(sop_engine *e) {
+  if (!e) return;
+  if (e->serve) sop_serve_stop(e);
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &a : e->bead_arrays) if (*a.first) (void)hipFree(*a.first);
+  for (auto &a : e->nat_arrays) if (*a.first) (void)hipFree(*a.first);
+  for (auto &a : e->cell_arrays) if (*a.first) (void)hipFree(*a.first);
+  void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.tile_sys, e->p.sys, e->p.sysout, e->p.syslist, e->p.work, e->p.nwork};
+  for (void *q : others) if (q) (void)hipFree(q);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char *sop_last_error(const sop_engine *e) { return e ? e->err.c_str() : "no engine"; }
+
+sop_status sop_synchronize(sop_engine *e) {
+  if (!e) return SOP_ERR_INVALID;
+  return hip_check(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+}
+
+static bool valid_sys(sop_engine *e, int32_t s) {
+  return e && s >= 0 && s < (int32_t)e->sys.size() && e->sys[s].used;
+}
+
+sop_status sop_system_add(sop_engine *e, const sop_system_desc *d, int32_t *sys) {
+  if (!e || !sys) return SOP_ERR_INVALID;
+  if (e->serve) return set_error(e, SOP_ERR_STATE, "engine is serving");
+  return engine_add_system(e, d, sys);
+}
+
+sop_status sop_system_remove(sop_engine *e, int32_t sys) {
+  if (!e) return SOP_ERR_INVALID;
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_remove_system(e, sys);
+}
+
+sop_status sop_system_info(sop_engine *e, int32_t sys, int32_t *n_beads, int64_t *step) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  if (n_beads) *n_beads = e->sys[sys].n;
+  if (step) *step = e->sys[sys].step;
+  return SOP_OK;
+}
+
+sop_status sop_update_neighbor_list(sop_engine *e, int32_t sys) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  sop_status st = engine_set_batch(e, &sys, 1);
+  if (st) return st;
+  HIPCK(launch_nl(e->p, grid_tiles(e), grid_sys(e), 0, 1, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, &sys, 1);
+}
+
+sop_status sop_update_pair_list(sop_engine *e, int32_t sys) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  sop_status st = engine_set_batch(e, &sys, 1);
+  if (st) return st;
+  HIPCK(launch_pl(e->p, grid_tiles(e), 0, 1, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, &sys, 1);
+}
+
+sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, int64_t cap, int64_t *n) {
+  if (!valid_sys(e, sys) || !n) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  HIPCK(hipStreamSynchronize(e->stream));
+  const SysHost &h = e->sys[sys];
+  const int32_t nb = h.n;
+  std::vector<int32_t> cnt((size_t)nb);
+  std::vector<int32_t> ent;
+  std::vector<int32_t> start((size_t)nb);
+  int32_t *cnt_d = nullptr;
+  int64_t seg_len = 0;
+  if (which == SOP_LIST_NL_NATIVE || which == SOP_LIST_PL_NATIVE) {
+    cnt_d = which == SOP_LIST_NL_NATIVE ? e->p.nlatt_cnt : e->p.platt_cnt;
+    HIPCK(hipMemcpy(start.data(), e->p.nat_start + h.bead_off, 4 * (size_t)nb, hipMemcpyDeviceToHost));
+    ent.resize((size_t)h.nat_len);
+    HIPCK(hipMemcpy(ent.data(), (which == SOP_LIST_NL_NATIVE ? e->p.nlatt_j : e->p.platt_j) + h.nat_off,
+                    4 * (size_t)h.nat_len, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; ++i) start[i] -= (int32_t)h.nat_off;
+  } else if (which == SOP_LIST_NL_REP || which == SOP_LIST_PL_REP) {
+    const bool nl = which == SOP_LIST_NL_REP;
+    cnt_d = nl ? e->p.nlrep_cnt : e->p.plrep_cnt;
+    seg_len = nl ? h.nl_len : h.pl_len;
+    ent.resize((size_t)seg_len);
+    HIPCK(hipMemcpy(ent.data(), (nl ? e->p.nlrep_j + h.nl_off : e->p.plrep_j + h.pl_off), 4 * (size_t)seg_len,
+                    hipMemcpyDeviceToHost));
+    const int32_t capb = nl ? h.nl_cap : h.pl_cap;
+    for (int i = 0; i < nb; ++i) start[i] = i * capb;
+  } else {
+    return set_error(e, SOP_ERR_INVALID, "unknown list");
+  }
+  HIPCK(hipMemcpy(cnt.data(), cnt_d + h.bead_off, 4 * (size_t)nb, hipMemcpyDeviceToHost));
+  int64_t m = 0;
+  for (int i = 0; i < nb; ++i)
+    for (int k = 0; k < cnt[i]; ++k) {
+      const int32_t j = ent[(size_t)start[i] + k] - (int32_t)h.bead_off;
+      if (j > i) {
+        if (ij && m < cap) {
+          ij[2 * m] = i;
+          ij[2 * m + 1] = j;
+        }
+        ++m;
+      }
+    }
+  *n = m;
+  return SOP_OK;
+}
+
+sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *stt) {
+  if (!valid_sys(e, sys) || !stt) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  HIPCK(hipStreamSynchronize(e->stream));
+  const SysHost &h = e->sys[sys];
+  std::vector<int32_t> c((size_t)h.n);
+  int64_t tot[4];
+  int32_t mx[4];
+  int32_t *arrs[4] = {e->p.nlatt_cnt, e->p.nlrep_cnt, e->p.platt_cnt, e->p.plrep_cnt};
+  for (int q = 0; q < 4; ++q) {
+    HIPCK(hipMemcpy(c.data(), arrs[q] + h.bead_off, 4 * (size_t)h.n, hipMemcpyDeviceToHost));
+    tot[q] = 0;
+    mx[q] = 0;
+    for (int32_t v : c) { tot[q] += v; mx[q] = std::max(mx[q], v); }
+  }
+  stt->nl_native = tot[0] / 2;
+  stt->nl_rep = tot[1] / 2;
+  stt->pl_native = tot[2] / 2;
+  stt->pl_rep = tot[3] / 2;
+  stt->nl_max_per_bead = mx[1];
+  stt->pl_max_per_bead = mx[3];
+  return SOP_OK;
+}
+
+sop_status sop_force_eval(sop_engine *e, int32_t sys, int32_t with_noise) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  sop_status st = engine_set_batch(e, &sys, 1);
+  if (st) return st;
+  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, grid_tiles(e), 0,
+                    e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, &sys, 1);
+}
+
+sop_status sop_get_forces(sop_engine *e, int32_t sys, float *f3n) {
+  if (!valid_sys(e, sys) || !f3n) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  HIPCK(hipStreamSynchronize(e->stream));
+  const SysHost &h = e->sys[sys];
+  std::vector<float4> f((size_t)h.n);
+  HIPCK(hipMemcpy(f.data(), e->p.fout + h.bead_off, 16 * (size_t)h.n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < h.n; ++i) {
+    f3n[3 * i] = f[i].x;
+    f3n[3 * i + 1] = f[i].y;
+    f3n[3 * i + 2] = f[i].z;
+  }
+  return SOP_OK;
+}
+
+static sop_status eval_energy(sop_engine *e, int32_t sys, double *out) {
+  sop_status st = engine_set_batch(e, &sys, 1);
+  if (st) return st;
+  HIPCK(launch_energy(e->p, grid_tiles(e), grid_sys(e), e->stream));
+  HIPCK(hipMemcpyAsync(out, e->p.sysout + (int64_t)sys * SYSOUT_N, 8 * SYSOUT_N, hipMemcpyDeviceToHost, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return SOP_OK;
+}
+
+sop_status sop_energy_eval(sop_engine *e, int32_t sys, double *energies) {
+  if (!valid_sys(e, sys) || !energies) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  double o[SYSOUT_N];
+  sop_status st = eval_energy(e, sys, o);
+  if (st) return st;
+  for (int k = 0; k < SOP_E_COUNT; ++k) energies[k] = o[k];
+  return SOP_OK;
+}
+
+sop_status sop_observables(sop_engine *e, int32_t sys, double *Q, int32_t *contacts, double *rgsq) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  double o[SYSOUT_N];
+  sop_status st = eval_energy(e, sys, o);
+  if (st) return st;
+  const int32_t c = (int32_t)std::llround(o[6]);
+  if (contacts) *contacts = c;
+  if (Q) *Q = e->sys[sys].n_nat > 0 ? (double)c / (double)e->sys[sys].n_nat : 0.0;
+  if (rgsq) *rgsq = o[7];
+  return SOP_OK;
+}
+
+static sop_status check_run_args(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
+  if (!e || !sys || nsys <= 0 || nsteps < 0) return set_error(e, SOP_ERR_INVALID, "bad run arguments");
+  if (e->serve) return set_error(e, SOP_ERR_STATE, "engine is serving");
+  for (int q = 0; q < nsys; ++q) {
+    if (!valid_sys(e, sys[q])) return set_error(e, SOP_ERR_INVALID, "no such system");
+    for (int r = 0; r < q; ++r)
+      if (sys[r] == sys[q]) return set_error(e, SOP_ERR_INVALID, "system listed twice");
+  }
+  return SOP_OK;
+}
+
+static void mirror_advance(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
+  for (int q = 0; q < nsys; ++q) {
+    e->sys[sys[q]].step += nsteps;
+    e->sys[sys[q]].parity = (int32_t)((e->sys[sys[q]].parity + nsteps) & 1);
+  }
+}
+
+sop_status sop_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
+  sop_status st = check_run_args(e, sys, nsys, nsteps);
+  if (st) return st;
+  if (nsteps == 0) return SOP_OK;
+  st = engine_set_batch(e, sys, nsys);
+  if (st) return st;
+  HIPCK(launch_begin(e->p, nsteps, e->stream));
+  // graph blocks need a uniform schedule and phase
+  const sop_params &P0 = e->sys[sys[0]].params;
+  bool uniform = e->opts.use_graphs != 0 && P0.n_nl % P0.n_pl == 0;
+  const int64_t G = P0.n_nl;
+  const int64_t r0 = e->sys[sys[0]].step % G;
+  for (int q = 1; q < nsys && uniform; ++q) {
+    const SysHost &h = e->sys[sys[q]];
+    uniform = h.params.n_nl == P0.n_nl && h.params.n_pl == P0.n_pl && h.step % G == r0;
+  }
+  int64_t left = nsteps;
+  if (uniform && G > 1) {
+    const int64_t pre = std::min(left, (G - r0) % G);
+    if (pre > 0) {
+      st = engine_run_hostloop(e, pre);
+      if (st) return st;
+      mirror_advance(e, sys, nsys, pre);
+      left -= pre;
+    }
+    while (left >= G) {
+      st = engine_run_block_graph(e, (int32_t)G);
+      if (st) return st;
+      mirror_advance(e, sys, nsys, G);
+      left -= G;
+    }
+  }
+  if (left > 0) {
+    st = engine_run_hostloop(e, left);
+    if (st) return st;
+    mirror_advance(e, sys, nsys, left);
+  }
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, sys, nsys);
+}
+
+sop_status sop_run_plain(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
+  sop_status st = check_run_args(e, sys, nsys, nsteps);
+  if (st) return st;
+  for (int q = 0; q < nsys; ++q) {
+    st = engine_set_batch(e, &sys[q], 1);
+    if (st) return st;
+    HIPCK(launch_begin(e->p, nsteps, e->stream));
+    st = engine_run_hostloop(e, nsteps);
+    if (st) return st;
+    mirror_advance(e, &sys[q], 1, nsteps);
+  }
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, sys, nsys);
+}
+
+static sop_status read_coords(sop_engine *e, int32_t sys, std::vector<float4> &x) {
+  SysDev S;
+  sop_status st = engine_read_sys(e, sys, &S);
+  if (st) return st;
+  const SysHost &h = e->sys[sys];
+  x.resize((size_t)h.n);
+  HIPCK(hipMemcpy(x.data(), (S.parity & 1 ? e->p.pos1 : e->p.pos0) + h.bead_off, 16 * (size_t)h.n,
+                  hipMemcpyDeviceToHost));
+  return SOP_OK;
+}
+
+sop_status sop_get_coords(sop_engine *e, int32_t sys, float *xyz, int64_t *step) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  std::vector<float4> x;
+  sop_status st = read_coords(e, sys, x);
+  if (st) return st;
+  if (xyz)
+    for (size_t i = 0; i < x.size(); ++i) {
+      xyz[3 * i] = x[i].x;
+      xyz[3 * i + 1] = x[i].y;
+      xyz[3 * i + 2] = x[i].z;
+    }
+  if (step) {
+    SysDev S;
+    st = engine_read_sys(e, sys, &S);
+    if (st) return st;
+    *step = S.step;
+  }
+  return SOP_OK;
+}
+
+sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz) {
+  if (!valid_sys(e, sys) || !xyz) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  std::vector<float4> x;
+  sop_status st = read_coords(e, sys, x);
+  if (st) return st;
+  const float boxl = e->sys[sys].params.boxl;
+  for (size_t i = 0; i < x.size(); ++i) {
+    float v[3] = {x[i].x, x[i].y, x[i].z};
+    for (int d = 0; d < 3; ++d) {
+      if (boxl > 0.0f) v[d] = v[d] - boxl * rintf(v[d] / boxl);
+      xyz[3 * i + d] = v[d];
+    }
+  }
+  return SOP_OK;
+}
+
+sop_status sop_set_coords(sop_engine *e, int32_t sys, const float *xyz) {
+  if (!valid_sys(e, sys) || !xyz) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  SysDev S;
+  sop_status st = engine_read_sys(e, sys, &S);
+  if (st) return st;
+  const SysHost &h = e->sys[sys];
+  std::vector<float4> x((size_t)h.n);
+  for (int i = 0; i < h.n; ++i) {
+    float wf;
+    std::memcpy(&wf, &h.wbits[i], 4);
+    x[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], wf);
+  }
+  HIPCK(hipMemcpy((S.parity & 1 ? e->p.pos1 : e->p.pos0) + h.bead_off, x.data(), 16 * (size_t)h.n,
+                  hipMemcpyHostToDevice));
+  return SOP_OK;
+}
+
+sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t reps, double *avg_ms) {
+  sop_status st = check_run_args(e, sys, nsys, reps);
+  if (st) return st;
+  if (reps <= 0 || !avg_ms) return set_error(e, SOP_ERR_INVALID, "reps > 0 and avg_ms required");
+  st = engine_set_batch(e, sys, nsys);
+  if (st) return st;
+  HIPCK(launch_begin(e->p, reps, e->stream));
+  hipEvent_t a, b;
+  HIPCK(hipEventCreate(&a));
+  HIPCK(hipEventCreate(&b));
+  const int gt = grid_tiles(e), lpb = e->opts.lanes_per_bead;
+  HIPCK(hipEventRecord(a, e->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, gt, r, e->stream));
+  HIPCK(hipEventRecord(b, e->stream));
+  HIPCK(launch_advance(e->p, reps, e->stream));
+  HIPCK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIPCK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  mirror_advance(e, sys, nsys, reps);
+  *avg_ms = ms / reps;
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, sys, nsys);
+}
+
+sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes, int32_t *grid) {
+  if (!e) return SOP_ERR_INVALID;
+  const int32_t saved_l = e->opts.lanes_per_bead, saved_g = e->opts.grid_blocks;
+  double best = 1e30;
+  int32_t bl = saved_l, bg = saved_g;
+  for (int32_t l : {8, 16, 32}) {
+    for (int32_t gm : {4, 8, 16}) {
+      e->opts.lanes_per_bead = l;
+      e->opts.grid_blocks = gm * e->num_cu;
+      double ms = 0.0;
+      sop_status st = sop_time_step_kernel(e, sys, nsys, 3, &ms);  // warm
+      if (st == SOP_OK) st = sop_time_step_kernel(e, sys, nsys, 10, &ms);
+      if (st) {
+        e->opts.lanes_per_bead = saved_l;
+        e->opts.grid_blocks = saved_g;
+        return st;
+      }
+      if (ms < best) {
+        best = ms;
+        bl = l;
+        bg = gm * e->num_cu;
+      }
+    }
+  }
+  e->opts.lanes_per_bead = bl;
+  e->opts.grid_blocks = bg;
+  invalidate_graphs(e);
+  if (lanes) *lanes = bl;
+  if (grid) *grid = bg;
+  return SOP_OK;
+}
+
+// Device self test for the tests: per-lane failure bits of the wave helpers
+// and 64 x 3 device normals for (seed 0x9abcdef012345678, bead = lane,
+// step = 1000 + lane).
+sop_status sop_selftest(sop_engine *e, int32_t *bad64, float *normals192) {
+  if (!e) return SOP_ERR_INVALID;
+  int *d_out = nullptr;
+  float *d_f = nullptr;
+  HIPCK(hipMalloc((void **)&d_out, 64 * 4));
+  HIPCK(hipMalloc((void **)&d_f, 64 * 12));
+  HIPCK(launch_selftest(d_out, d_f, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  HIPCK(hipMemcpy(bad64, d_out, 64 * 4, hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(normals192, d_f, 64 * 12, hipMemcpyDeviceToHost));
+  (void)hipFree(d_out);
+  (void)hipFree(d_f);
+  return SOP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,113 @@
+// engine.h -- internal declarations of the libsop host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sop.h"
+#include "sop_dev.h"
+
+namespace sopk {
+
+hipError_t launch_nl(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int force_all,
+                     hipStream_t st);
+hipError_t launch_pl(const DevPtrs &p, int grid_tiles, int kstep, int force_all, hipStream_t st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid_tiles, int kstep,
+                       hipStream_t st);
+hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st);
+hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st);
+hipError_t launch_begin(const DevPtrs &p, int64_t nsteps, hipStream_t st);
+hipError_t launch_selftest(int *out, float *fout, hipStream_t st);
+
+// First-fit range allocator over [0, cap).
+class RangeAlloc {
+ public:
+  void init(int64_t cap);
+  int64_t alloc(int64_t n, int64_t align); // -1 when full
+  void release(int64_t off, int64_t n);
+  void grow(int64_t new_cap);
+  int64_t cap() const { return cap_; }
+  int64_t used() const { return used_; }
+
+ private:
+  int64_t cap_ = 0, used_ = 0;
+  std::map<int64_t, int64_t> free_;
+};
+
+// Device array that can grow, keeping its contents.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct SysHost {
+  bool used = false;
+  int32_t n = 0, n_nat = 0;
+  int64_t bead_off = 0, nat_off = 0, nl_off = 0, pl_off = 0, cell_off = 0;
+  int64_t bead_len = 0, nat_len = 0, nl_len = 0, pl_len = 0, cell_len = 0;
+  int32_t nl_cap = 0, pl_cap = 0, cell_cap = 0;
+  int64_t step = 0;  // host mirror (exact outside serving)
+  int32_t parity = 0;
+  sop_params params{};
+  std::vector<int32_t> wbits;  // w component of each bead (chain<<2|type)
+};
+
+struct Pools {
+  int64_t beads = 0, nat = 0, nl = 0, pl = 0, cells = 0, slots = 0;
+};
+
+} // namespace sopk
+
+struct sop_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  sop_engine_opts opts{};
+  std::string err;
+  bool fixed_pools = false;
+
+  sopk::Pools cap;
+  sopk::RangeAlloc a_beads, a_nat, a_nl, a_pl, a_cells;
+  std::vector<sopk::SysHost> sys;  // by slot
+  sopk::DevPtrs p{};
+  // raw buffers in allocation order (for growth/free)
+  std::vector<std::pair<void **, size_t>> bead_arrays, nat_arrays, cell_arrays;
+
+  // work tables (host staging)
+  std::vector<int32_t> h_work, h_syslist;
+  int64_t work_cap = 0, syslist_cap = 0;
+  int32_t cur_nwork = 0, cur_nsys = 0;
+
+  // graphs keyed by block shape
+  std::map<std::string, hipGraphExec_t> graphs;
+
+  // serving
+  struct Job;
+  struct Serve;
+  Serve *serve = nullptr;
+};
+
+namespace sopk {
+// engine internals shared with serve.cpp
+sop_status set_error(sop_engine *e, sop_status st, const std::string &msg);
+sop_status hip_check(sop_engine *e, hipError_t h, const char *what);
+sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *slot);
+sop_status engine_remove_system(sop_engine *e, int32_t slot);
+sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots);
+sop_status engine_run_block_graph(sop_engine *e, int32_t G);
+sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps);
+sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslots);
+sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out);
+int grid_tiles(const sop_engine *e);
+int grid_sys(const sop_engine *e);
+int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d);
+} // namespace sopk

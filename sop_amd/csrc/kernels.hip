@@ -1,0 +1,949 @@
+// kernels.hip -- gfx950 kernels of the SOP overdamped hot path.
+//
+//   k_bbox / k_cell_count / k_cell_scan / k_cell_scatter / k_cell_rank /
+//   k_nl_build   neighbor-list rebuild: per-system open cell grid, counting
+//                sort of beads into cells (histogram, DPP wave scan, scatter,
+//                deterministic in-cell ranking), 27-cell stencil with the exact
+//                FP32 predicate and ballot/mbcnt wave compaction.  Replaces the
+//                reference's flag kernels + CUDPP split/scan over the O(N^2)
+//                master list (sop.cu:704-826, 928-1262, 1827-2608).
+//   k_pl_build   pair-list filter of the neighbor list, same compaction
+//                (sop.cu:835-920, 1456-1620).
+//   k_step       fused FENE + i,i+2 soft sphere + native LJ + non-native
+//                repulsion gather (no atomics), group reduction over the lanes
+//                of a bead, Philox4x32-10 random force, overdamped move
+//                (energy.cu:81-167,228-604; sop.cu:487-520).
+//   k_energy / k_sys_reduce   FP64 reductions of FP32 pair energies, virial,
+//                native contacts and Rg^2 (energy.cu:194-442, sop.cu:528-611).
+//
+// Every kernel walks a device-resident work table with a grid-stride loop, so
+// one launch covers any batch of systems and a captured graph stays valid when
+// the batch changes.
+#include <hip/hip_runtime.h>
+
+#include "sop_dev.h"
+
+namespace sopk {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ int w_type(float w) { return __float_as_int(w) & 3; }
+__device__ __forceinline__ int w_chain(float w) { return __float_as_int(w) >> 2; }
+
+__device__ __forceinline__ const float4 *buf_in(const DevPtrs &p, int parity) {
+  return (parity & 1) ? p.pos1 : p.pos0;
+}
+__device__ __forceinline__ float4 *buf_out(const DevPtrs &p, int parity) {
+  return (parity & 1) ? p.pos0 : p.pos1;
+}
+
+// Exact FP32 predicate pieces, same operation order and rounding as the
+// reference's single-precision list kernels; this file is compiled with
+// -ffp-contract=off and correctly rounded division/sqrt.
+__device__ __forceinline__ float mi_exact(float d, float boxl) {
+  if (boxl > 0.0f) {
+    float q = d / boxl;
+    float r = rintf(q);
+    float pr = boxl * r;
+    d = d - pr;
+  }
+  return d;
+}
+
+__device__ __forceinline__ float dist2_exact(float4 a, float4 b, float boxl) {
+  float dx = mi_exact(b.x - a.x, boxl);
+  float dy = mi_exact(b.y - a.y, boxl);
+  float dz = mi_exact(b.z - a.z, boxl);
+  float p = dx * dx;
+  float q = dy * dy;
+  float r = dz * dz;
+  return (p + q) + r;
+}
+
+__device__ __forceinline__ float cutoff_of(double factor, float absval, float base) {
+  return factor > 0.0 ? (float)(factor * (double)base) : absval;
+}
+
+// Fast minimum image for forces (not used by any list predicate).
+__device__ __forceinline__ float mi_fast(float d, float boxl, float inv_boxl) {
+  return d - boxl * rintf(d * inv_boxl);
+}
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// DPP lane moves.  Controls: quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E,
+// row_half_mirror = 0x141, row_mirror = 0x140, row_shr:n = 0x110+n,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// Sum over aligned groups of W lanes; every lane of a group receives the sum.
+template <int W>
+__device__ __forceinline__ float group_sum(float v) {
+  static_assert(W == 4 || W == 8 || W == 16 || W == 32 || W == 64, "group width");
+  v += dpp_f<0xB1>(v);
+  v += dpp_f<0x4E>(v);
+  if (W >= 8) v += dpp_f<0x141>(v);
+  if (W >= 16) v += dpp_f<0x140>(v);
+  if (W >= 32) {
+    // xor 16 inside each 32-lane half: ds_swizzle bit mode, and=0x1F, xor=0x10
+    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  }
+  if (W >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave with DPP: Hillis-Steele
+// inside each 16-lane row (row_shr 1,2,4,8), then row_bcast15 / row_bcast31
+// carry row totals into the rows above.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
+  return v;
+}
+
+// Number of set bits of mask in lanes below this one (ballot prefix).
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 24-bit uniform in (0,1): odd multiples of 2^-24, exact in float.
+__device__ __forceinline__ float u24(uint32_t r) {
+  return (float)((r >> 8) | 1u) * 5.9604644775390625e-08f;
+}
+
+// Three standard normals for (seed, bead, step): Box-Muller on Philox output.
+__device__ __forceinline__ void normals3(uint32_t k0, uint32_t k1, uint32_t bead, int64_t step,
+                                         float &n0, float &n1, float &n2) {
+  const uint4 r = philox10(make_uint4(bead, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), 0u), k0, k1);
+  const float ra = sqrtf(-2.0f * logf(u24(r.x)));
+  const float rb = sqrtf(-2.0f * logf(u24(r.z)));
+  float sa, ca, sb, cb;
+  sincospif(2.0f * u24(r.y), &sa, &ca);
+  sincospif(2.0f * u24(r.w), &sb, &cb);
+  n0 = ra * ca;
+  n1 = ra * sa;
+  n2 = rb * cb;
+}
+
+__device__ __forceinline__ bool live(const SysDev &S, int kstep) { return S.step + kstep < S.step_end; }
+__device__ __forceinline__ bool due_nl(const SysDev &S, int kstep) {
+  return live(S, kstep) && ((S.step + kstep) % S.n_nl) == 0;
+}
+__device__ __forceinline__ bool due_pl(const SysDev &S, int kstep) {
+  return live(S, kstep) && ((S.step + kstep) % S.n_pl) == 0;
+}
+
+__device__ __forceinline__ int cell_index(const SysDev &S, float4 x) {
+  int cx = (int)floorf((x.x - S.org[0]) * S.inv_edge[0]);
+  int cy = (int)floorf((x.y - S.org[1]) * S.inv_edge[1]);
+  int cz = (int)floorf((x.z - S.org[2]) * S.inv_edge[2]);
+  cx = min(max(cx, 0), S.dims[0] - 1);
+  cy = min(max(cy, 0), S.dims[1] - 1);
+  cz = min(max(cz, 0), S.dims[2] - 1);
+  return (cz * S.dims[1] + cy) * S.dims[0] + cx;
+}
+
+// ------------------------------------------------- neighbor-list rebuild
+
+// Per system: bounding box -> open cell grid; zero the cell histogram.
+__global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep, int force_all) {
+  __shared__ float red[6][4];
+  __shared__ int s_ncells;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
+    const int s = p.syslist[q];
+    SysDev &S = p.sys[s];
+    if (!(force_all || due_nl(S, kstep))) continue;
+    const float4 *pin = buf_in(p, S.parity + kstep);
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = tid; b < S.n; b += blockDim.x) {
+      const float4 x = pin[S.bead_off + b];
+      mn[0] = fminf(mn[0], x.x); mx[0] = fmaxf(mx[0], x.x);
+      mn[1] = fminf(mn[1], x.y); mx[1] = fmaxf(mx[1], x.y);
+      mn[2] = fminf(mn[2], x.z); mx[2] = fmaxf(mx[2], x.z);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = fminf(mn[d], __shfl_xor(mn[d], o));
+        mx[d] = fmaxf(mx[d], __shfl_xor(mx[d], o));
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) { red[d][wave] = mn[d]; red[3 + d][wave] = mx[d]; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float ext_max = 0.0f;
+      int nc = 1;
+      for (int d = 0; d < 3; ++d) {
+        float a = red[d][0], b = red[3 + d][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { a = fminf(a, red[d][w]); b = fmaxf(b, red[3 + d][w]); }
+        const float ext = b - a;
+        int dims = 1;
+        if (ext > 0.0f) dims = (int)floorf(ext / S.cell_min_edge);
+        dims = min(max(dims, 1), S.dmax);
+        S.org[d] = a;
+        S.inv_edge[d] = ext > 0.0f ? (float)dims / ext : 0.0f;
+        S.dims[d] = dims;
+        nc *= dims;
+        ext_max = fmaxf(ext_max, ext);
+      }
+      S.ncells = nc;
+      if (S.boxl > 0.0f) {
+        const int ok = ext_max < 0.5f * S.boxl - S.wrap_margin;
+        S.wrap_free = ok;
+        if (!ok) atomicOr(&S.err, ERR_PBC);
+      } else {
+        S.wrap_free = 1;
+      }
+      s_ncells = nc;
+    }
+    __syncthreads();
+    const int nc = s_ncells;
+    for (int c = tid; c < nc; c += blockDim.x) p.cell_cnt[S.cell_base + c] = 0;
+    __syncthreads();
+  }
+}
+
+// Per bead: cell id and histogram.
+__global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep, int force_all) {
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const SysDev &S = p.sys[p.tile_sys[t]];
+    if (!(force_all || due_nl(S, kstep))) continue;
+    const int i = t * TILE + threadIdx.x;
+    if (i < S.bead_off + S.n) {
+      const int c = cell_index(S, buf_in(p, S.parity + kstep)[i]);
+      p.cell_of[i] = c;
+      atomicAdd(&p.cell_cnt[S.cell_base + c], 1);
+    }
+  }
+}
+
+// Per system: exclusive scan of the histogram (DPP wave scan + wave totals).
+__global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p, int kstep, int force_all) {
+  __shared__ int wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
+    const int s = p.syslist[q];
+    SysDev &S = p.sys[s];
+    if (!(force_all || due_nl(S, kstep))) continue;
+    const int nc = S.ncells, cb = S.cell_base;
+    int carry = 0;
+    for (int c0 = 0; c0 < nc; c0 += 256) {
+      const int c = c0 + tid;
+      const int v = c < nc ? p.cell_cnt[cb + c] : 0;
+      const int incl = wave_incl_scan(v);
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      int off = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        off += (w < wave) ? wsum[w] : 0;
+        tot += wsum[w];
+      }
+      if (c < nc) {
+        p.cell_start[cb + c] = carry + off + incl - v;
+        p.cell_cursor[cb + c] = 0;
+      }
+      carry += tot;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      p.cell_start[cb + nc] = carry;
+      if (carry != S.n) atomicOr(&S.err, ERR_INTERNAL);
+    }
+  }
+}
+
+// Per bead: scatter into its cell's segment (arbitrary in-cell order).
+__global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p, int kstep, int force_all) {
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const SysDev &S = p.sys[p.tile_sys[t]];
+    if (!(force_all || due_nl(S, kstep))) continue;
+    const int i = t * TILE + threadIdx.x;
+    if (i < S.bead_off + S.n) {
+      const int c = p.cell_of[i];
+      const int slot = atomicAdd(&p.cell_cursor[S.cell_base + c], 1);
+      const int pos = p.cell_start[S.cell_base + c] + slot;
+      if (pos >= 0 && pos < S.n) p.cell_tmp[S.bead_off + pos] = i;
+    }
+  }
+}
+
+// Per bead: rank among the beads of its cell by index -> sorted segment.
+// Makes the in-cell order, and with it every list, independent of atomics.
+__global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep, int force_all) {
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const SysDev &S = p.sys[p.tile_sys[t]];
+    if (!(force_all || due_nl(S, kstep))) continue;
+    const int i = t * TILE + threadIdx.x;
+    if (i < S.bead_off + S.n) {
+      const int c = p.cell_of[i];
+      const int st = p.cell_start[S.cell_base + c], en = p.cell_start[S.cell_base + c + 1];
+      int rank = 0;
+      if (st >= 0 && en <= S.n) {
+        for (int m = st; m < en; ++m) rank += p.cell_tmp[S.bead_off + m] < i;
+        const int pos = st + rank;
+        if (pos < S.n) p.cell_sorted[S.bead_off + pos] = i;
+      }
+    }
+  }
+}
+
+// Wave per bead: native neighbor list from the bead's native table, and the
+// non-native neighbor list from the 27 surrounding cells.
+__global__ __launch_bounds__(256) void k_nl_build(DevPtrs p, int kstep, int force_all) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const int s = p.tile_sys[t];
+    const SysDev &S = p.sys[s];
+    if (!(force_all || due_nl(S, kstep))) continue;
+    const float4 *pin = buf_in(p, S.parity + kstep);
+    const float boxl = S.boxl;
+    const int bend = min(t * TILE + TILE, S.bead_off + S.n);
+    for (int q = 0; q < TILE / 4; ++q) {
+      const int i = t * TILE + wave * (TILE / 4) + q;
+      if (i >= bend) break;
+      const float4 xi = pin[i];
+      const int wi = __float_as_int(xi.w), ti = wi & 3, ci = wi >> 2;
+      const int ns = p.nat_start[i], nn = p.nat_cnt[i];
+      // natives
+      int cnt = 0;
+      for (int k0 = 0; k0 < nn; k0 += 64) {
+        const int k = k0 + lane;
+        bool pred = false;
+        int j = 0;
+        float r0 = 0.0f;
+        if (k < nn) {
+          j = p.nat_j[ns + k];
+          r0 = p.nat_r0[ns + k];
+          const float rc = cutoff_of(S.nl_factor, S.nl_abs, r0);
+          pred = dist2_exact(xi, pin[j], boxl) < rc * rc;
+        }
+        const uint64_t m = __ballot(pred);
+        if (pred) {
+          const int o = ns + cnt + lanes_below(m);
+          p.nlatt_j[o] = j;
+          p.nlatt_r0[o] = r0;
+        }
+        cnt += __popcll(m);
+      }
+      if (lane == 0) p.nlatt_cnt[i] = cnt;
+      // non-natives
+      const int cx = min(max((int)floorf((xi.x - S.org[0]) * S.inv_edge[0]), 0), S.dims[0] - 1);
+      const int cy = min(max((int)floorf((xi.y - S.org[1]) * S.inv_edge[1]), 0), S.dims[1] - 1);
+      const int cz = min(max((int)floorf((xi.z - S.org[2]) * S.inv_edge[2]), 0), S.dims[2] - 1);
+      const int64_t base = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
+      const float *rc2row = &S.rc2_nl_rep[(ti - 1) * 2];
+      int cr = 0;
+      for (int zz = max(cz - 1, 0); zz <= min(cz + 1, S.dims[2] - 1); ++zz)
+        for (int yy = max(cy - 1, 0); yy <= min(cy + 1, S.dims[1] - 1); ++yy)
+          for (int xx = max(cx - 1, 0); xx <= min(cx + 1, S.dims[0] - 1); ++xx) {
+            const int c = (zz * S.dims[1] + yy) * S.dims[0] + xx;
+            const int st = max(p.cell_start[S.cell_base + c], 0);
+            const int en = min(p.cell_start[S.cell_base + c + 1], S.n);
+            for (int m0 = st; m0 < en; m0 += 64) {
+              const int m = m0 + lane;
+              bool pred = false;
+              int j = 0;
+              if (m < en) j = p.cell_sorted[S.bead_off + m];
+              if (m < en && j >= S.bead_off && j < S.bead_off + S.n) {
+                const float4 xj = pin[j];
+                const int wj = __float_as_int(xj.w);
+                const int dj = j - i;
+                const bool excl = (j == i) || (((wj >> 2) == ci) && dj >= -2 && dj <= 2);
+                if (!excl) {
+                  const float d2 = dist2_exact(xi, xj, boxl);
+                  if (d2 < rc2row[(wj & 3) - 1]) {
+                    bool nat = false;
+                    for (int k = 0; k < nn; ++k) nat |= (p.nat_j[ns + k] == j);
+                    pred = !nat;
+                  }
+                }
+              }
+              const uint64_t mk = __ballot(pred);
+              if (pred) {
+                const int o = cr + lanes_below(mk);
+                if (o < S.nl_cap) p.nlrep_j[base + o] = j;
+              }
+              cr += __popcll(mk);
+            }
+          }
+      if (lane == 0) {
+        p.nlrep_cnt[i] = min(cr, S.nl_cap);
+        if (cr > S.nl_cap) atomicOr(&p.sys[s].err, ERR_NL_OVF);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------ pair-list filter
+
+__global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep, int force_all) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const int s = p.tile_sys[t];
+    const SysDev &S = p.sys[s];
+    if (!(force_all || due_pl(S, kstep))) continue;
+    const float4 *pin = buf_in(p, S.parity + kstep);
+    const float boxl = S.boxl;
+    const int bend = min(t * TILE + TILE, S.bead_off + S.n);
+    for (int q = 0; q < TILE / 4; ++q) {
+      const int i = t * TILE + wave * (TILE / 4) + q;
+      if (i >= bend) break;
+      const float4 xi = pin[i];
+      const int ti = __float_as_int(xi.w) & 3;
+      const int ns = p.nat_start[i];
+      const int na = p.nlatt_cnt[i];
+      int cnt = 0;
+      for (int k0 = 0; k0 < na; k0 += 64) {
+        const int k = k0 + lane;
+        bool pred = false;
+        int j = 0;
+        float r0 = 0.0f;
+        if (k < na) {
+          j = p.nlatt_j[ns + k];
+          r0 = p.nlatt_r0[ns + k];
+          const float rc = cutoff_of(S.pl_factor, S.pl_abs, r0);
+          pred = dist2_exact(xi, pin[j], boxl) < rc * rc;
+        }
+        const uint64_t m = __ballot(pred);
+        if (pred) {
+          const int o = ns + cnt + lanes_below(m);
+          p.platt_j[o] = j;
+          p.platt_r0[o] = r0;
+        }
+        cnt += __popcll(m);
+      }
+      if (lane == 0) p.platt_cnt[i] = cnt;
+      const int64_t nb = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
+      const int64_t pb = S.pl_base + (int64_t)(i - S.bead_off) * S.pl_cap;
+      const int nr = p.nlrep_cnt[i];
+      const float *rc2row = &S.rc2_pl_rep[(ti - 1) * 2];
+      int cr = 0;
+      for (int m0 = 0; m0 < nr; m0 += 64) {
+        const int m = m0 + lane;
+        bool pred = false;
+        int j = 0;
+        if (m < nr) {
+          j = p.nlrep_j[nb + m];
+          const float4 xj = pin[j];
+          pred = dist2_exact(xi, xj, boxl) < rc2row[w_type(xj.w) - 1];
+        }
+        const uint64_t mk = __ballot(pred);
+        if (pred) {
+          const int o = cr + lanes_below(mk);
+          if (o < S.pl_cap) p.plrep_j[pb + o] = j;
+        }
+        cr += __popcll(mk);
+      }
+      if (lane == 0) {
+        p.plrep_cnt[i] = min(cr, S.pl_cap);
+        if (cr > S.pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------- fused force + integrate
+
+// LPB lanes gather one bead's partners; a 256-thread block covers one tile of
+// 64 beads in LPB/4 rounds, stages the 64 reduced forces in LDS, and wave 0
+// then draws the random force and moves all 64 beads (one bead per lane).
+template <int LPB, int MODE>
+__global__ __launch_bounds__(256) void k_step(DevPtrs p, int kstep) {
+  __shared__ float4 sF[TILE];
+  constexpr int GPW = 64 / LPB;              // bead groups per wave
+  constexpr int ROUNDS = TILE / (4 * GPW);   // rounds to cover the tile
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane / LPB, l = lane % LPB;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const int s = p.tile_sys[t];
+    const SysDev &S = p.sys[s];
+    const int64_t step = S.step + kstep;
+    const int bead_off = S.bead_off, n = S.n;
+    const int b0 = t * TILE;
+    const int bend = min(b0 + TILE, bead_off + n);
+    const int par = S.parity + kstep;
+    const float4 *__restrict__ pin = buf_in(p, par);
+    float4 *__restrict__ pout = buf_out(p, par);
+    if (MODE == MODE_STEP && step >= S.step_end) {
+      // finished inside this block: carry the coordinates to the other buffer
+      if (tid < TILE && b0 + tid < bend) pout[b0 + tid] = pin[b0 + tid];
+      continue;
+    }
+    const float boxl = S.boxl, inv_boxl = S.inv_boxl;
+    const bool wrapf = S.wrap_free != 0 || boxl <= 0.0f;
+    for (int r = 0; r < ROUNDS; ++r) {
+      const int lb = r * (4 * GPW) + wave * GPW + g;
+      const int i = b0 + lb;
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      if (i < bend) {
+        const float4 xi = pin[i];
+        const int wi = __float_as_int(xi.w);
+        const int row = ((wi & 3) - 1) * 2;
+        // chain partners i-2, i-1, i+1, i+2: FENE bonds and i,i+2 soft sphere
+        if (l < 4) {
+          const int off = l < 2 ? l - 2 : l - 1;
+          const int j = i + off;
+          if (j >= bead_off && j < bead_off + n) {
+            const float4 xj = pin[j];
+            if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
+              float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+              if (!wrapf) {
+                dx = mi_fast(dx, boxl, inv_boxl);
+                dy = mi_fast(dy, boxl, inv_boxl);
+                dz = mi_fast(dz, boxl, inv_boxl);
+              }
+              const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              float c;
+              if (off == 1 || off == -1) {
+                const float r0 = p.bond_r0[off < 0 ? j : i];
+                const float d = sqrtf(d2);
+                const float dev = d - r0;
+                const float gg = 1.0f - dev * dev * S.inv_R0sq;
+                c = S.k_bnd * dev * rcpf(d * gg);
+                if (gg <= 0.0f) atomicOr(&p.sys[s].err, ERR_BOND);
+              } else {
+                const float ir2 = rcpf(d2);
+                const float ir4 = ir2 * ir2;
+                c = -S.ss_f6 * (ir4 * ir4);
+              }
+              fx = fmaf(c, dx, fx);
+              fy = fmaf(c, dy, fy);
+              fz = fmaf(c, dz, fz);
+            }
+          }
+        }
+        // native LJ over the native pair list
+        {
+          const int ns = p.nat_start[i], nc = p.platt_cnt[i];
+          const float eA = S.eps_att[row], eB = S.eps_att[row + 1];
+          for (int k = l; k < nc; k += LPB) {
+            const int j = p.platt_j[ns + k];
+            const float r0 = p.platt_r0[ns + k];
+            const float4 xj = pin[j];
+            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            if (!wrapf) {
+              dx = mi_fast(dx, boxl, inv_boxl);
+              dy = mi_fast(dy, boxl, inv_boxl);
+              dz = mi_fast(dz, boxl, inv_boxl);
+            }
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float eps = w_type(xj.w) == 2 ? eB : eA;
+            const float r02 = r0 * r0;
+            const float ir2 = rcpf(d2);
+            const float q2 = r02 * ir2;
+            const float q6 = q2 * q2 * q2;
+            float c = -12.0f * eps * ir2 * q6 * (q6 - 1.0f);
+            c = d2 > 1.0e-7f * r02 ? c : 0.0f;
+            fx = fmaf(c, dx, fx);
+            fy = fmaf(c, dy, fy);
+            fz = fmaf(c, dz, fz);
+          }
+        }
+        // non-native repulsion over the non-native pair list
+        {
+          const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
+          const int nr = p.plrep_cnt[i];
+          const float s2A = S.sig2_rep[row], s2B = S.sig2_rep[row + 1];
+          const float m12A = S.rep_m12[row], m12B = S.rep_m12[row + 1];
+          const float m6A = S.rep_m6[row], m6B = S.rep_m6[row + 1];
+          const float gA = S.rep_guard[row], gB = S.rep_guard[row + 1];
+          const int32_t *__restrict__ lst = p.plrep_j + pb;
+          for (int k = l; k < nr; k += LPB) {
+            const int j = lst[k];
+            const float4 xj = pin[j];
+            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            if (!wrapf) {
+              dx = mi_fast(dx, boxl, inv_boxl);
+              dy = mi_fast(dy, boxl, inv_boxl);
+              dz = mi_fast(dz, boxl, inv_boxl);
+            }
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const bool tb = w_type(xj.w) == 2;
+            const float s2 = tb ? s2B : s2A;
+            const float ir2 = rcpf(d2);
+            const float q2 = s2 * ir2;
+            const float q6 = q2 * q2 * q2;
+            float c = ir2 * q6 * fmaf(tb ? m12B : m12A, q6, tb ? m6B : m6A);
+            c = d2 > (tb ? gB : gA) ? c : 0.0f;
+            fx = fmaf(c, dx, fx);
+            fy = fmaf(c, dy, fy);
+            fz = fmaf(c, dz, fz);
+          }
+        }
+      }
+      fx = group_sum<LPB>(fx);
+      fy = group_sum<LPB>(fy);
+      fz = group_sum<LPB>(fz);
+      if (l == 0 && i < bend) sF[lb] = make_float4(fx, fy, fz, 0.0f);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int i = b0 + lane;
+      if (i < bend) {
+        const float4 f = sF[lane];
+        if (MODE == MODE_FORCE) {
+          p.fout[i] = f;
+        } else {
+          float n0, n1, n2;
+          normals3(S.seed_lo, S.seed_hi, (uint32_t)(i - bead_off), step, n0, n1, n2);
+          const float var = S.var;
+          const float ftx = f.x + n0 * var, fty = f.y + n1 * var, ftz = f.z + n2 * var;
+          if (MODE == MODE_FORCE_NOISE) {
+            p.fout[i] = make_float4(ftx, fty, ftz, 0.0f);
+          } else {
+            float4 x = pin[i];
+            const float a5 = S.a5;
+            x.x = x.x + a5 * ftx;
+            x.y = x.y + a5 * fty;
+            x.z = x.z + a5 * ftz;
+            pout[i] = x;
+            if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------ energies / observables
+
+// 16 lanes per bead; FP32 pair energies, FP64 accumulation; pairs are seen
+// from both beads and weighted 1/2.  Native contacts (d/r0 < 1.25 over all
+// natives) are counted from the lower index.
+__global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
+  constexpr int LPB = 16, GPW = 4, ROUNDS = TILE / 16;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane / LPB, l = lane % LPB;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const int s = p.tile_sys[t];
+    const SysDev &S = p.sys[s];
+    const int bead_off = S.bead_off, n = S.n;
+    const int b0 = t * TILE;
+    const int bend = min(b0 + TILE, bead_off + n);
+    const float4 *pin = buf_in(p, S.parity);
+    const float boxl = S.boxl, inv_boxl = S.inv_boxl;
+    const bool wrapf = S.wrap_free != 0 || boxl <= 0.0f;
+    for (int r = 0; r < ROUNDS; ++r) {
+      const int i = b0 + r * (4 * GPW) + wave * GPW + g;
+      double eb = 0.0, ea = 0.0, en = 0.0, er = 0.0, vir = 0.0, con = 0.0;
+      if (i < bend) {
+        const float4 xi = pin[i];
+        const int wi = __float_as_int(xi.w);
+        const int row = ((wi & 3) - 1) * 2;
+        if (l < 4) {
+          const int off = l < 2 ? l - 2 : l - 1;
+          const int j = i + off;
+          if (j >= bead_off && j < bead_off + n) {
+            const float4 xj = pin[j];
+            if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
+              float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+              if (!wrapf) {
+                dx = mi_fast(dx, boxl, inv_boxl);
+                dy = mi_fast(dy, boxl, inv_boxl);
+                dz = mi_fast(dz, boxl, inv_boxl);
+              }
+              const float d2 = dx * dx + dy * dy + dz * dz;
+              if (off == 1 || off == -1) {
+                const float r0 = p.bond_r0[off < 0 ? j : i];
+                const float d = sqrtf(d2);
+                const float dev = d - r0;
+                const float gg = 1.0f - dev * dev * S.inv_R0sq;
+                eb += 0.5 * (double)(-S.half_k_R0sq * log1pf(-dev * dev * S.inv_R0sq));
+                const float c = S.k_bnd * dev / (d * gg);
+                vir += 0.5 * (double)(-c * d2);
+              } else {
+                const float ir2 = 1.0f / d2;
+                const float ir6 = ir2 * ir2 * ir2;
+                ea += 0.5 * (double)(S.ss_e6 * ir6);
+                vir += 0.5 * (double)(S.ss_f6 * ir6);
+              }
+            }
+          }
+        }
+        const int ns = p.nat_start[i];
+        const int nc = p.platt_cnt[i];
+        for (int k = l; k < nc; k += LPB) {
+          const int j = p.platt_j[ns + k];
+          const float r0 = p.platt_r0[ns + k];
+          const float4 xj = pin[j];
+          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+          if (!wrapf) {
+            dx = mi_fast(dx, boxl, inv_boxl);
+            dy = mi_fast(dy, boxl, inv_boxl);
+            dz = mi_fast(dz, boxl, inv_boxl);
+          }
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          const float eps = S.eps_att[row + w_type(xj.w) - 1];
+          const float r02 = r0 * r0;
+          const float q2 = r02 / d2;
+          const float q6 = q2 * q2 * q2;
+          en += 0.5 * (double)(eps * (q6 * q6 - 2.0f * q6));
+          if (d2 > 1.0e-7f * r02) vir += 0.5 * (double)(12.0f * eps * q6 * (q6 - 1.0f));
+        }
+        // contacts over all natives, counted once (partner index above i)
+        const int nn = p.nat_cnt[i];
+        for (int k = l; k < nn; k += LPB) {
+          const int j = p.nat_j[ns + k];
+          if (j > i) {
+            const float d = sqrtf(dist2_exact(xi, pin[j], boxl));
+            if (d / p.nat_r0[ns + k] < 1.25f) con += 1.0;
+          }
+        }
+        const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
+        const int nr = p.plrep_cnt[i];
+        for (int k = l; k < nr; k += LPB) {
+          const int j = p.plrep_j[pb + k];
+          const float4 xj = pin[j];
+          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+          if (!wrapf) {
+            dx = mi_fast(dx, boxl, inv_boxl);
+            dy = mi_fast(dy, boxl, inv_boxl);
+            dz = mi_fast(dz, boxl, inv_boxl);
+          }
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          const int tt = row + w_type(xj.w) - 1;
+          const float q2 = S.sig2_rep[tt] / d2;
+          const float q6 = q2 * q2 * q2;
+          er += 0.5 * (double)(S.eps_rep[tt] * (S.c12 * q6 * q6 + S.c6 * q6));
+          if (d2 > S.rep_guard[tt])
+            vir += 0.5 * (double)(-(S.rep_m12[tt] * q6 + S.rep_m6[tt]) * q6);
+        }
+      }
+#pragma unroll
+      for (int o = LPB / 2; o >= 1; o >>= 1) {
+        eb += __shfl_xor(eb, o);
+        ea += __shfl_xor(ea, o);
+        en += __shfl_xor(en, o);
+        er += __shfl_xor(er, o);
+        vir += __shfl_xor(vir, o);
+        con += __shfl_xor(con, o);
+      }
+      if (l == 0 && i < bend) {
+        double *o = p.epart + (int64_t)i * EPART_N;
+        o[0] = eb; o[1] = ea; o[2] = en; o[3] = er; o[4] = vir; o[5] = con;
+      }
+    }
+  }
+}
+
+// Per system, fixed-order reduction of the bead partials plus Rg^2 from the
+// centroid form (1/N) sum |r|^2 - |r_cm|^2, identical to the reference's
+// pair sum / N^2 (sop.cu:564-580).
+__global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs p) {
+  __shared__ double sh[4][10];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
+    const int s = p.syslist[q];
+    const SysDev &S = p.sys[s];
+    const float4 *pin = buf_in(p, S.parity);
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+    for (int b = tid; b < S.n; b += blockDim.x) {
+      const int i = S.bead_off + b;
+      const double *e = p.epart + (int64_t)i * EPART_N;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k] += e[k];
+      const float4 x = pin[i];
+      acc[6] += x.x;
+      acc[7] += x.y;
+      acc[8] += x.z;
+      acc[9] += (double)x.x * x.x + (double)x.y * x.y + (double)x.z * x.z;
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc[k] = wave_sum_d(acc[k]);
+    if (lane == 0)
+      for (int k = 0; k < 10; ++k) sh[wave][k] = acc[k];
+    __syncthreads();
+    if (tid == 0) {
+      double tot[10];
+      for (int k = 0; k < 10; ++k) tot[k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
+      double *o = p.sysout + (int64_t)s * SYSOUT_N;
+      o[0] = tot[0]; o[1] = tot[1]; o[2] = tot[2]; o[3] = tot[3];
+      o[4] = tot[0] + tot[1] + tot[2] + tot[3];
+      o[5] = tot[4];
+      o[6] = tot[5];
+      const double inv = 1.0 / (double)S.n;
+      const double cx = tot[6] * inv, cy = tot[7] * inv, cz = tot[8] * inv;
+      o[7] = tot[9] * inv - (cx * cx + cy * cy + cz * cz);
+      o[8] = tot[6]; o[9] = tot[7]; o[10] = tot[8];
+      o[11] = (double)S.n;
+    }
+    __syncthreads();
+  }
+}
+
+// Per system: open a run of nsteps (the step kernel moves a system while
+// step < step_end).
+__global__ void k_begin(DevPtrs p, int64_t nsteps) {
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nsys; q += gridDim.x * blockDim.x) {
+    SysDev &S = p.sys[p.syslist[q]];
+    S.step_end = S.step + nsteps;
+  }
+}
+
+// Per system: advance the step counter after a block of kernel steps.
+__global__ void k_advance(DevPtrs p, int nsteps) {
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nsys; q += gridDim.x * blockDim.x) {
+    SysDev &S = p.sys[p.syslist[q]];
+    const int64_t st = S.step + nsteps;
+    S.step = st < S.step_end ? st : (S.step_end > S.step ? S.step_end : S.step);
+    S.parity = (S.parity + nsteps) & 1;
+  }
+}
+
+// ------------------------------------------------------------ self test
+// Lane-layout checks of the DPP / swizzle helpers against shuffles, and of
+// the device normal generator; run by the GPU tests before the engine.
+__global__ void k_selftest(int *out, float *fout) {
+  const int lane = threadIdx.x;
+  const int v = (lane * 7 + 3) % 11;
+  int ref = 0;
+  for (int k = 0; k <= lane; ++k) ref += (k * 7 + 3) % 11;
+  const int sc = wave_incl_scan(v);
+  int bad = sc != ref;
+  const float fv = (float)((lane * 5) % 13);
+  float r4 = 0, r8 = 0, r16 = 0, r32 = 0, r64 = 0;
+  for (int k = 0; k < 64; ++k) {
+    const float x = (float)((k * 5) % 13);
+    if (k / 4 == lane / 4) r4 += x;
+    if (k / 8 == lane / 8) r8 += x;
+    if (k / 16 == lane / 16) r16 += x;
+    if (k / 32 == lane / 32) r32 += x;
+    r64 += x;
+  }
+  bad |= (group_sum<4>(fv) != r4) << 1;
+  bad |= (group_sum<8>(fv) != r8) << 2;
+  bad |= (group_sum<16>(fv) != r16) << 3;
+  bad |= (group_sum<32>(fv) != r32) << 4;
+  bad |= (group_sum<64>(fv) != r64) << 5;
+  const uint64_t m = __ballot(lane % 3 == 0);
+  int below = 0;
+  for (int k = 0; k < lane; ++k) below += (k % 3 == 0);
+  bad |= (lanes_below(m) != below) << 6;
+  out[lane] = bad;
+  float n0, n1, n2;
+  normals3(0x12345678u, 0x9abcdef0u, (uint32_t)lane, 1000 + lane, n0, n1, n2);
+  fout[3 * lane] = n0;
+  fout[3 * lane + 1] = n1;
+  fout[3 * lane + 2] = n2;
+}
+
+// --------------------------------------------------------------- launchers
+
+#define SOPK_LAUNCH_CHECK() hipGetLastError()
+
+hipError_t launch_nl(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int force_all,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep, force_all);
+  hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, st, p, kstep, force_all);
+  hipLaunchKernelGGL(k_cell_scan, dim3(grid_sys), dim3(256), 0, st, p, kstep, force_all);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_tiles), dim3(64), 0, st, p, kstep, force_all);
+  hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, st, p, kstep, force_all);
+  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep, force_all);
+  return SOPK_LAUNCH_CHECK();
+}
+
+hipError_t launch_pl(const DevPtrs &p, int grid_tiles, int kstep, int force_all, hipStream_t st) {
+  hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep, force_all);
+  return SOPK_LAUNCH_CHECK();
+}
+
+template <int MODE>
+static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid_tiles, int kstep, hipStream_t st) {
+  switch (lpb) {
+    case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
+    case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
+    case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
+    default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
+  }
+  return SOPK_LAUNCH_CHECK();
+}
+
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid_tiles, int kstep, hipStream_t st) {
+  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, grid_tiles, kstep, st);
+  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, grid_tiles, kstep, st);
+  return launch_step_mode<MODE_STEP>(p, lpb, grid_tiles, kstep, st);
+}
+
+hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy, dim3(grid_tiles), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_sys_reduce, dim3(grid_sys), dim3(256), 0, st, p);
+  return SOPK_LAUNCH_CHECK();
+}
+
+hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(4), dim3(256), 0, st, p, nsteps);
+  return SOPK_LAUNCH_CHECK();
+}
+
+hipError_t launch_begin(const DevPtrs &p, int64_t nsteps, hipStream_t st) {
+  hipLaunchKernelGGL(k_begin, dim3(4), dim3(256), 0, st, p, nsteps);
+  return SOPK_LAUNCH_CHECK();
+}
+
+hipError_t launch_selftest(int *out, float *fout, hipStream_t st) {
+  hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, st, out, fout);
+  return SOPK_LAUNCH_CHECK();
+}
+
+} // namespace sopk

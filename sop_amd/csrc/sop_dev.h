@@ -1,0 +1,95 @@
+// sop_dev.h -- device-resident state shared by the HIP kernels and the host
+// engine.  One SysDev per resident system (replica); per-bead and per-entry
+// arrays live in pooled arenas addressed through DevPtrs.
+#pragma once
+
+#include <stdint.h>
+
+namespace sopk {
+
+// Beads per tile.  Every system's bead range starts on a tile boundary, so a
+// tile never spans two systems and tile t covers arena beads [64t, 64t+64).
+constexpr int TILE = 64;
+
+// SysDev::err bits (sticky until the host clears them)
+constexpr int ERR_NL_OVF = 1;    // a bead's neighbor list exceeded nl_cap
+constexpr int ERR_PL_OVF = 2;    // a bead's pair list exceeded pl_cap
+constexpr int ERR_PBC = 4;       // extent too close to boxl/2 for the open cell grid
+constexpr int ERR_NAN = 8;       // non-finite coordinate after a step
+constexpr int ERR_BOND = 16;     // FENE bond stretched to |r - r0| >= R0
+constexpr int ERR_INTERNAL = 32; // cell counts inconsistent
+
+// kernel modes of the fused force kernel
+constexpr int MODE_STEP = 0;        // forces + random force + overdamped move
+constexpr int MODE_FORCE = 1;       // deterministic forces -> fout
+constexpr int MODE_FORCE_NOISE = 2; // forces + random force -> fout
+
+struct SysDev {
+  // arena placement
+  int32_t bead_off;  // arena index of bead 0 (multiple of TILE)
+  int32_t n;         // beads
+  int32_t nl_cap;    // per-bead capacity of the non-native neighbor list
+  int32_t pl_cap;    // per-bead capacity of the non-native pair list
+  int64_t nl_base;   // first entry of this system's neighbor-list segment
+  int64_t pl_base;   // first entry of this system's pair-list segment
+  int32_t cell_base; // first cell of this system's cell segment (cell_cap+1 slots)
+  int32_t cell_cap;
+  int32_t dmax;      // max cells per axis, dmax^3 <= cell_cap
+  int32_t parity;    // which coordinate buffer holds the current coordinates
+  // schedule
+  int64_t step;      // completed steps
+  int64_t step_end;  // the step kernel moves the system only while step < step_end
+  int32_t n_nl, n_pl;
+  // dynamics
+  float a5, var;     // h/zeta and sqrt(2 T zeta / h)
+  float boxl, inv_boxl;
+  uint32_t seed_lo, seed_hi;
+  // force field (type-pair tables indexed (ti-1)*2 + (tj-1))
+  float k_bnd, inv_R0sq, half_k_R0sq;
+  float ss_e6, ss_f6; // eps_ss*sigma_ss^6 and 6*eps_ss*sigma_ss^6
+  float eps_att[4];
+  float eps_rep[4], sig2_rep[4], rep_guard[4], rep_m12[4], rep_m6[4];
+  float c12, c6;
+  float rc2_nl_rep[4], rc2_pl_rep[4];
+  double nl_factor, pl_factor;
+  float nl_abs, pl_abs;
+  float cell_min_edge; // >= largest neighbor-list cutoff, with a 0.1% margin
+  float wrap_margin;
+  // cell grid, written by k_bbox at each neighbor-list rebuild
+  float org[3];
+  float inv_edge[3];
+  int32_t dims[3];
+  int32_t ncells;
+  int32_t wrap_free; // 1: every pair separation is below boxl/2 (minimum image is the identity)
+  // status, device-written
+  int32_t err;
+  int32_t pad_[2];
+};
+
+struct DevPtrs {
+  SysDev *sys;
+  float4 *pos0, *pos1;  // double-buffered unwrapped coordinates; w = bits (chain<<2 | type)
+  float *bond_r0;       // r0 of bond (i, i+1)
+  int32_t *nat_start, *nat_cnt;   // native CSR per bead (two-sided, sorted by partner)
+  int32_t *nat_j;       // partner (arena index)
+  float *nat_r0;
+  int32_t *nlatt_j; float *nlatt_r0; int32_t *nlatt_cnt; // native neighbor list (same slots)
+  int32_t *platt_j; float *platt_r0; int32_t *platt_cnt; // native pair list (same slots)
+  int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list
+  int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list
+  int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
+  int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
+  float4 *fout;         // forces (force-eval modes)
+  double *epart;        // per bead: 6 energy partials
+  double *sysout;       // per system slot: SYSOUT_N reduced values
+  int32_t *tile_sys;    // system slot of each tile
+  int32_t *work;        // tile ids to process (-1 = padding), XCD-interleaved
+  int32_t *nwork;
+  int32_t *syslist;     // system slots of the current batch
+  int32_t *nsys;
+};
+
+constexpr int EPART_N = 6;   // bond, angle, native, rep, virial, contacts
+constexpr int SYSOUT_N = 16; // e[0..5], contacts, rgsq, sum x,y,z, n
+
+} // namespace sopk

@@ -1,0 +1,171 @@
+"""C ABI and host-side logic of libsop (CPU only: no compute call needs a GPU).
+
+- libsop.so loads and exports every function include/sop.h declares;
+- without a device the engine refuses to start (no CPU fallback);
+- parameter sets, the seeded chain generator, native maps, and the readers /
+  writers of the reference's file formats (io.cpp:112-408).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sop_amd as S
+from sop_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", h)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(sop_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(_abi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_covers_the_header():
+    assert set(declared_functions()) == set(_abi.SIGNATURES)
+
+
+def test_struct_sizes_match_c_layout():
+    # offsets follow the C declaration order with natural alignment
+    assert C.sizeof(_abi.Params) == 200
+    assert _abi.Params.nl_factor.offset == 136 and _abi.Params.seed.offset == 192
+    assert C.sizeof(_abi.SystemDesc) == 8 * 9 + 200 + 8
+
+
+def test_engine_without_device_fails_loudly():
+    h = C.c_void_p()
+    st = S.lib().sop_engine_create(0, None, C.byref(h))
+    if st == _abi.SOP_OK:  # running on a GPU host
+        S.lib().sop_engine_destroy(h)
+        pytest.skip("a device is present")
+    assert st == _abi.SOP_ERR_NO_DEVICE
+    with pytest.raises(S.SopError):
+        S.Engine(0)
+
+
+def test_parameter_sets():
+    r = S.params_reference()
+    assert (r.k_bnd, r.R0, r.sigma_ss, r.n_nl, r.n_pl, r.nl_factor, r.pl_factor) == (20, 2, 3.5, 50, 1, 3.2, 2.5)
+    assert r.eps_att[1][1] == pytest.approx(1.0) and r.eps_att[2][2] == pytest.approx(0.7)
+    assert r.sigma_rep[1][2] == pytest.approx(5.4) and r.boxl == 500 and r.seed == 1234
+    c = S.params_config()
+    assert (c.nl_abs, c.pl_abs, c.n_nl, c.n_pl, c.zeta, c.h, c.T) == (32, 16, 50, 10, 50, pytest.approx(0.05),
+                                                                      pytest.approx(0.6))
+    assert c.sigma_ss == pytest.approx(5.4) and c.eps_att[1][1] == 1 and c.boxl == 0
+    a5, var = S.params_derived(c)
+    assert a5 == np.float32(np.float32(0.05) / np.float32(50.0))
+    assert var == np.float32(np.sqrt(2.0 * np.float64(np.float32(0.6)) * 50.0 / np.float64(np.float32(0.05))))
+
+
+@pytest.mark.parametrize("n", [10, 128, 2000])
+def test_generator_is_seeded_compact_and_self_avoiding(n):
+    from scipy.spatial import cKDTree
+    x1, b1 = S.generate_chain(5, n)
+    x2, b2 = S.generate_chain(5, n)
+    x3, _ = S.generate_chain(6, n)
+    assert np.array_equal(x1, x2) and np.array_equal(b1, b2) and not np.array_equal(x1, x3)
+    d = np.linalg.norm(np.diff(x1.astype(np.float64), axis=0), axis=1)
+    assert np.allclose(d, 3.8, atol=1e-5) and np.allclose(b1[:-1], d, atol=1e-6)
+    pairs = cKDTree(x1).query_pairs(4.0 - 1e-4, output_type="ndarray")
+    far = pairs[np.abs(pairs[:, 0] - pairs[:, 1]) >= 2]
+    assert len(far) == 0
+    rg = np.sqrt(((x1 - x1.mean(0)) ** 2).sum(1).mean())
+    R = (3 * n * 140.0 / (4 * np.pi)) ** (1 / 3) + 3.8
+    assert rg < R
+
+
+def test_native_contacts_match_brute_force():
+    x, _ = S.generate_chain(3, 300)
+    ch = np.repeat([0, 1, 2], 100).astype(np.int32)
+    for intra in (False, True):
+        ni, nj, nr = S.native_contacts(x, ch, 8.0, intra_only=intra)
+        d = np.linalg.norm(x[:, None, :].astype(np.float64) - x[None, :, :], axis=2)
+        i, j = np.triu_indices(300, 1)
+        ok = d[i, j] < 8.0
+        ok &= ~((ch[i] == ch[j]) & (j - i <= 2))
+        if intra:
+            ok &= ch[i] == ch[j]
+        want = set(zip(i[ok].tolist(), j[ok].tolist()))
+        assert set(zip(ni.tolist(), nj.tolist())) == want
+        assert np.allclose(nr, d[ni, nj].astype(np.float32))
+
+
+def write_reference_files(tmp, sysd):
+    """Writes bonds/angles/vdw/init in the reference formats (io.cpp:112-259)."""
+    n = sysd.n
+    with open(os.path.join(tmp, "bonds.dat"), "w") as f:
+        f.write(f"nbnd {n - 1}\n")
+        for i in range(n - 1):
+            f.write(f"{i + 1} {i + 2} {sysd.bond_r0[i]:.6f}\n")
+    with open(os.path.join(tmp, "angles.dat"), "w") as f:
+        f.write(f"nang {n - 2}\n")
+        for i in range(n - 2):
+            f.write(f"{i + 1} {i + 2} {i + 3} 0.0\n")
+    nat = {(int(a), int(b)): float(r) for a, b, r in zip(sysd.nat_i, sysd.nat_j, sysd.nat_r0)}
+    x = sysd.xyz.astype(np.float64)
+    lines = []
+    for i in range(n):
+        for j in range(i + 3, n):
+            r = nat.get((i, j), max(8.5, float(np.linalg.norm(x[i] - x[j]))))
+            lines.append(f"{i + 1} {j + 1} {r:.6f} 1 1\n")
+    with open(os.path.join(tmp, "vdw.dat"), "w") as f:
+        f.write(f"ncon_att {len(nat)} ncon_rep {len(lines) - len(nat)}\n")
+        f.writelines(lines)
+    with open(os.path.join(tmp, "init.dat"), "w") as f:
+        f.write(f"nbead {n}\n")
+        for i in range(n):
+            f.write(f"{i + 1} {x[i, 0]:.6f} {x[i, 1]:.6f} {x[i, 2]:.6f}\n")
+
+
+def test_reference_input_files_round_trip(tmp_path):
+    sysd = S.make_chain_system(4, 60)
+    write_reference_files(str(tmp_path), sysd)
+    t = S.System.from_reference_files(*(str(tmp_path / f) for f in ("bonds.dat", "angles.dat", "vdw.dat", "init.dat")))
+    assert t.n == 60 and np.all(t.chain == 0) and np.all(t.type == 1)
+    assert np.allclose(t.xyz, sysd.xyz, atol=1e-5)
+    assert np.allclose(t.bond_r0[:-1], sysd.bond_r0[:-1], atol=1e-5)
+    got = sorted(zip(t.nat_i.tolist(), t.nat_j.tolist()))
+    assert got == sorted(zip(sysd.nat_i.tolist(), sysd.nat_j.tolist()))
+    assert t.params.nl_factor == 3.2 and t.params.n_pl == 1
+
+
+def test_reference_reader_rejects_partial_master_list(tmp_path):
+    sysd = S.make_chain_system(4, 30)
+    write_reference_files(str(tmp_path), sysd)
+    p = tmp_path / "vdw.dat"
+    lines = p.read_text().splitlines()
+    head = lines[0].split()
+    head[3] = str(int(head[3]) - 1)
+    p.write_text(" ".join(head) + "\n" + "\n".join(lines[1:-1]) + "\n")
+    with pytest.raises(S.SopError):
+        S.System.from_reference_files(*(str(tmp_path / f) for f in ("bonds.dat", "angles.dat", "vdw.dat", "init.dat")))
+
+
+def test_trajectory_and_coordinate_writers(tmp_path):
+    x = np.array([[1.5, -2.25, 3.0], [4.0, 5.0, -6.125]], np.float32)
+    tr = str(tmp_path / "traj.bin")
+    p = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    assert S.lib().sop_write_traj(tr.encode(), 2, p(x)) == 0
+    assert S.lib().sop_write_traj(tr.encode(), 2, p(x)) == 0  # appends (io.cpp:270)
+    assert open(tr).read() == "1.500000 -2.250000 3.000000\n4.000000 5.000000 -6.125000\n" * 2
+    cf = str(tmp_path / "coord.out")
+    assert S.lib().sop_save_coords(cf.encode(), 2, p(x)) == 0
+    assert open(cf).read() == "1 1.500000 -2.250000 3.000000\n2 4.000000 5.000000 -6.125000\n"
+    y = np.zeros_like(x)
+    assert S.lib().sop_load_coords(cf.encode(), 2, p(y)) == 0
+    assert np.array_equal(x, y)
