@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <sstream>
@@ -625,6 +627,7 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SOP_ERR_NO_DEVICE;
   if (device < 0 || device >= count) return SOP_ERR_NO_DEVICE;
+  (void)hipGetLastError();  // do not inherit an error left by earlier calls
   sop_engine *e = new sop_engine();
   sop_engine_opts_default(&e->opts);
   if (opts) {
@@ -652,7 +655,9 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     return SOP_ERR_HIP;
   }
   e->num_cu = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+  // A blocking stream: synchronous hipMemcpy calls on the null stream order
+  // after the engine's kernels, so host reads never race device writes.
+  if (hipStreamCreate(&e->stream) != hipSuccess) {
     delete e;
     return SOP_ERR_HIP;
   }
@@ -660,14 +665,20 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
   // Lane-layout self test of the DPP / swizzle / ballot helpers.
   int *d_out = nullptr;
   float *d_f = nullptr;
-  int h_out[64];
-  bool ok = hipMalloc((void **)&d_out, 64 * 4) == hipSuccess && hipMalloc((void **)&d_f, 64 * 12) == hipSuccess &&
-            launch_selftest(d_out, d_f, e->stream) == hipSuccess &&
-            hipMemcpy(h_out, d_out, 64 * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  int h_out[64] = {0};
+  hipError_t h1 = hipMalloc((void **)&d_out, 64 * 4);
+  hipError_t h2 = h1 == hipSuccess ? hipMalloc((void **)&d_f, 64 * 12) : h1;
+  hipError_t h3 = h2 == hipSuccess ? launch_selftest(d_out, d_f, e->stream) : h2;
+  hipError_t h3s = h3 == hipSuccess ? hipStreamSynchronize(e->stream) : h3;
+  hipError_t h4 = h3s == hipSuccess ? hipMemcpy(h_out, d_out, 64 * 4, hipMemcpyDeviceToHost) : h3s;
+  bool ok = h4 == hipSuccess;
   for (int k = 0; ok && k < 64; ++k) ok = h_out[k] == 0;
-  (void)hipFree(d_out);
-  (void)hipFree(d_f);
+  if (d_out) (void)hipFree(d_out);
+  if (d_f) (void)hipFree(d_f);
   if (!ok) {
+    std::fprintf(stderr, "sop_engine_create: self test failed (%s / %s / %s / %s, lane0 bits %d)\n",
+                 hipGetErrorString(h1), hipGetErrorString(h2), hipGetErrorString(h3), hipGetErrorString(h4),
+                 h_out[0]);
     sop_engine_destroy(e);
     return SOP_ERR_HIP;
   }
@@ -694,15 +705,22 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
 void sop_engine_destroy(sop_engine *e) {
   if (!e) return;
   if (e->serve) sop_serve_stop(e);
-  (void)hipSetDevice(e->device);
-  if (e->stream) (void)hipStreamSynchronize(e->stream);
-  for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto &a : e->bead_arrays) if (*a.first) (void)hipFree(*a.first);
-  for (auto &a : e->nat_arrays) if (*a.first) (void)hipFree(*a.first);
-  for (auto &a : e->cell_arrays) if (*a.first) (void)hipFree(*a.first);
+  hipError_t first = hipSuccess;
+  const char *what = "";
+  auto note = [&](hipError_t h, const char *w) {
+    if (h != hipSuccess && first == hipSuccess) { first = h; what = w; }
+  };
+  note(hipSetDevice(e->device), "hipSetDevice");
+  if (e->stream) note(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  for (auto &kv : e->graphs) note(hipGraphExecDestroy(kv.second), "hipGraphExecDestroy");
+  for (auto &a : e->bead_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(bead pool)");
+  for (auto &a : e->nat_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(native pool)");
+  for (auto &a : e->cell_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(cell pool)");
   void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.tile_sys, e->p.sys, e->p.sysout, e->p.syslist, e->p.work, e->p.nwork};
-  for (void *q : others) if (q) (void)hipFree(q);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  for (void *q : others) if (q) note(hipFree(q), "hipFree");
+  if (e->stream) note(hipStreamDestroy(e->stream), "hipStreamDestroy");
+  if (first != hipSuccess) std::fprintf(stderr, "sop_engine_destroy: %s: %s\n", what, hipGetErrorString(first));
+  (void)hipGetLastError();
   delete e;
 }
 
@@ -1012,6 +1030,9 @@ sop_status sop_set_coords(sop_engine *e, int32_t sys, const float *xyz) {
   }
   HIPCK(hipMemcpy((S.parity & 1 ? e->p.pos1 : e->p.pos0) + h.bead_off, x.data(), 16 * (size_t)h.n,
                   hipMemcpyHostToDevice));
+  // new coordinates clear the system's sticky error bits
+  const int32_t zero = 0;
+  HIPCK(hipMemcpy((char *)(e->p.sys + sys) + offsetof(SysDev, err), &zero, 4, hipMemcpyHostToDevice));
   return SOP_OK;
 }
 

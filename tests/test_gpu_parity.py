@@ -182,7 +182,7 @@ def test_explicit_list_updates_after_moving_beads(eng):
     sid = eng.add(s)
     try:
         rng = np.random.default_rng(5)
-        x = s.xyz + rng.normal(0, 0.8, s.xyz.shape).astype(np.float32)
+        x = s.xyz + rng.normal(0, 0.25, s.xyz.shape).astype(np.float32)
         eng.set_coords(sid, x)
         eng.update_neighbor_list(sid)
         eng.update_pair_list(sid)
@@ -255,7 +255,8 @@ def test_pair_list_overflow_is_reported_not_written():
 def test_overstretched_bond_is_reported():
     s = c1()
     x = s.xyz.copy()
-    x[64:] += np.array([3.0, 0, 0], np.float32)  # bond 63-64 stretched by ~3 A > R0
+    u = (x[64] - x[63]) / np.linalg.norm(x[64] - x[63])
+    x[64:] += (3.0 * u).astype(np.float32)  # bond 63-64 stretched by 3 A > R0
     s.xyz = x
     with S.Engine(0) as e:
         sid = e.add(s)
