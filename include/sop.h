@@ -185,7 +185,8 @@ typedef struct sop_list_stats {
 } sop_list_stats;
 sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *st);
 /* Time `reps` launches of the fused force+integrate kernel over the listed
- * systems with HIP events on the engine stream; avg_ms per launch. */
+ * systems with HIP events on the engine stream; avg_ms per launch.  The
+ * systems' coordinates and counters are restored afterwards. */
 sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t reps,
                                 double *avg_ms);
 /* Pick lanes-per-bead and grid size for the listed systems by timing. */
