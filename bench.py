@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--lds-beads", type=int, default=0, help="LDS-resident step for systems up to this size "
+                    "(0 = engine default, < 0 = off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -108,7 +110,7 @@ def main():
     beads = sum(s.n for s in systems)
     log(f"workload: {len(systems)} replicas, {beads} beads, generated in {time.perf_counter() - t:.1f}s")
 
-    eng = S.Engine(0, lanes_per_bead=args.lanes)
+    eng = S.Engine(0, lanes_per_bead=args.lanes, lds_beads_max=args.lds_beads)
     t = time.perf_counter()
     ids = [eng.add(s) for s in systems]
     log(f"resident on device in {time.perf_counter() - t:.1f}s")

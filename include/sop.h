@@ -116,6 +116,8 @@ typedef struct sop_engine_opts {
   int32_t lanes_per_bead; /* force-kernel lanes per bead: 4, 8, 16 or 32; 0 = autotune */
   int32_t grid_blocks;    /* persistent grid size; 0 = 8 per compute unit */
   int32_t use_graphs;     /* capture step blocks as hipGraphs (default 1) */
+  int32_t lds_beads_max;  /* systems up to this size run the step with LDS-resident
+                             coordinates (default 1024; < 0 disables) */
   int64_t pool_bytes;     /* serve: HBM budget for all pools; 0 = grow on demand */
 } sop_engine_opts;
 

@@ -183,7 +183,7 @@ class Engine:
     """One HIP device's engine (libsop).  Not thread-safe."""
 
     def __init__(self, device=0, lanes_per_bead=16, grid_blocks=0, use_graphs=True, nl_cap_max=0,
-                 pl_cap_max=0, cell_cap_max=0, pool_bytes=0):
+                 pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0):
         self._h = None
         o = _abi.EngineOpts()
         lib().sop_engine_opts_default(C.byref(o))
@@ -197,6 +197,8 @@ class Engine:
         if cell_cap_max:
             o.cell_cap_max = cell_cap_max
         o.pool_bytes = pool_bytes
+        if lds_beads_max:
+            o.lds_beads_max = lds_beads_max
         h = C.c_void_p()
         _check(lib().sop_engine_create(device, C.byref(o), C.byref(h)))
         self._h = h

@@ -60,7 +60,7 @@ class EngineOpts(C.Structure):
     _fields_ = [
         ("nl_cap_max", C.c_int32), ("pl_cap_max", C.c_int32), ("cell_cap_max", C.c_int32),
         ("lanes_per_bead", C.c_int32), ("grid_blocks", C.c_int32), ("use_graphs", C.c_int32),
-        ("pool_bytes", C.c_int64),
+        ("lds_beads_max", C.c_int32), ("pool_bytes", C.c_int64),
     ]
 
 

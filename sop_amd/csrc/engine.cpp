@@ -83,6 +83,13 @@ int grid_tiles(const sop_engine *e) {
 
 int grid_sys(const sop_engine *e) { return std::max(1, std::min(e->cur_nsys, 2048)); }
 
+int grid_items(const sop_engine *e) {
+  int g = e->opts.grid_blocks > 0 ? e->opts.grid_blocks : 8 * e->num_cu;
+  g = std::max(8, g / 8 * 8);
+  const int need = (e->cur_nitems + 7) / 8 * 8;
+  return std::max(8, std::min(g, need));
+}
+
 } // namespace sopk
 
 #define HIPCK(expr)                                            \
@@ -99,13 +106,15 @@ static void bind_arrays(sop_engine *e) {
   DevPtrs &p = e->p;
   e->bead_arrays = {
       {(void **)&p.pos0, 16},      {(void **)&p.pos1, 16},       {(void **)&p.bond_r0, 4},
+      {(void **)&p.pos_ref, 16},   {(void **)&p.pos_sorted, 16},
       {(void **)&p.nat_start, 4},  {(void **)&p.nat_cnt, 4},     {(void **)&p.nlatt_cnt, 4},
       {(void **)&p.platt_cnt, 4},  {(void **)&p.nlrep_cnt, 4},   {(void **)&p.plrep_cnt, 4},
       {(void **)&p.cell_of, 4},    {(void **)&p.cell_tmp, 4},    {(void **)&p.cell_sorted, 4},
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},   {(void **)&p.nat_r0, 4},   {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
-  e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4}};
+  e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
+                    {(void **)&p.cellitems, 4}};
   (void)bead_elem_bytes;
 }
 
@@ -189,6 +198,8 @@ static sop_status ensure_capacity(sop_engine *e, int64_t beads, int64_t nat, int
     s = grow_array(e, (void **)&e->p.sysout, 8 * SYSOUT_N * (size_t)c.slots, 8 * SYSOUT_N * (size_t)nn);
     if (s) return s;
     s = grow_array(e, (void **)&e->p.syslist, 4 * (size_t)c.slots, 4 * (size_t)nn);
+    if (s) return s;
+    s = grow_array(e, (void **)&e->p.cellwork, 4 * (size_t)(c.slots + 1), 4 * (size_t)(nn + 1));
     if (s) return s;
     c.slots = nn;
     e->sys.resize((size_t)nn);
@@ -430,6 +441,27 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   S.nl_abs = P.nl_abs;
   S.pl_abs = P.pl_abs;
   S.cell_min_edge = std::max(1.0e-3f, max_rc * 1.001f);
+  // pair-list grid and list skin (the direct pair-list build is exact while
+  // every bead stays within skin/2 of its neighbor-list snapshot)
+  float max_rcp = 0.0f, skin = INFINITY;
+  for (int q = 0; q < 4; ++q) {
+    const int ti = q / 2 + 1, tj = q % 2 + 1;
+    const float sg = P.sigma_rep[ti][tj];
+    const float rn = cutoff_host(P.nl_factor, P.nl_abs, sg), rp = cutoff_host(P.pl_factor, P.pl_abs, sg);
+    max_rcp = std::max(max_rcp, rp);
+    skin = std::min(skin, rn - rp);
+  }
+  for (int k = 0; k < nn; ++k) {
+    const float r0 = d->nat_r0[k];
+    skin = std::min(skin, cutoff_host(P.nl_factor, P.nl_abs, r0) - cutoff_host(P.pl_factor, P.pl_abs, r0));
+  }
+  S.cell_min_edge_pl = std::max(1.0e-3f, max_rcp * 1.001f);
+  S.skin = skin > 0.0f ? skin * (1.0f - 1.0e-5f) : 0.0f;
+  S.build = BUILD_NONE;
+  S.nl_valid = 0;
+  int tmask = 0;
+  for (int i = 0; i < n; ++i) tmask |= 1 << (type[i] - 1);
+  S.type_mask = tmask;
   S.wrap_margin = 10.0f;
   S.dims[0] = S.dims[1] = S.dims[2] = 1;
   S.ncells = 1;
@@ -439,8 +471,7 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   // initial lists (the reference builds both before its step loop)
   st = engine_set_batch(e, &slot, 1);
   if (st) return st;
-  HIPCK(launch_nl(e->p, grid_tiles(e), grid_sys(e), 0, 1, e->stream));
-  HIPCK(launch_pl(e->p, grid_tiles(e), 0, 1, e->stream));
+  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_NL | FORCE_PL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   st = engine_check_errors(e, &slot, 1);
   if (st) {
@@ -475,12 +506,15 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     if (e->sys[a].bead_len != e->sys[b].bead_len) return e->sys[a].bead_len > e->sys[b].bead_len;
     return a < b;
   });
-  std::vector<std::vector<int32_t>> bins(NX);
+  std::vector<std::vector<int32_t>> bins(NX), bsys(NX);
   std::vector<int64_t> load(NX, 0);
   for (int32_t s : order) {
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     const SysHost &h = e->sys[s];
-    for (int64_t t = h.bead_off / TILE; t < (h.bead_off + h.n + TILE - 1) / TILE; ++t) bins[x].push_back((int32_t)t);
+    for (int64_t t = h.bead_off / TILE; t < (h.bead_off + h.n + TILE - 1) / TILE; ++t) {
+      bins[x].push_back((int32_t)t);
+      bsys[x].push_back(s);
+    }
     load[x] += (h.n + TILE - 1) / TILE;
   }
   size_t mx = 0;
@@ -488,6 +522,34 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   e->h_work.assign(mx * NX, -1);
   for (int x = 0; x < NX; ++x)
     for (size_t m = 0; m < bins[x].size(); ++m) e->h_work[m * NX + x] = bins[x][m];
+  // step-kernel items: runs of up to ITEM_TILES_MAX tiles of one system, on the
+  // same XCD bin as the system's tiles
+  std::vector<std::vector<int4>> ibins(NX);
+  size_t lds = 0;
+  const int lds_max = e->opts.lds_beads_max;
+  for (int x = 0; x < NX; ++x) {
+    int32_t cur_sys = -1, first = 0, cnt = 0, flags = 0;
+    for (size_t m = 0; m < bins[x].size(); ++m) {
+      const int32_t t = bins[x][m], sys_of_t = bsys[x][m];
+      if (sys_of_t != cur_sys || cnt == ITEM_TILES_MAX || t != first + cnt) {
+        if (cur_sys >= 0) ibins[x].push_back(make_int4(cur_sys, first, cnt, flags));
+        cur_sys = sys_of_t;
+        first = t;
+        cnt = 0;
+        const int32_t n = e->sys[sys_of_t].n;
+        flags = (lds_max > 0 && n <= lds_max && (size_t)n * 16 <= e->lds_limit) ? ITEM_LDS : 0;
+        if (flags) lds = std::max(lds, (size_t)n * 16);
+      }
+      ++cnt;
+    }
+    if (cur_sys >= 0) ibins[x].push_back(make_int4(cur_sys, first, cnt, flags));
+  }
+  size_t imx = 0;
+  for (auto &b : ibins) imx = std::max(imx, b.size());
+  e->h_items.assign(imx * NX, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < NX; ++x)
+    for (size_t m = 0; m < ibins[x].size(); ++m) e->h_items[m * NX + x] = ibins[x][m];
+  e->cur_lds = lds;
   e->h_syslist.assign(slots, slots + nslots);
   if ((int64_t)e->h_work.size() + 2 > e->work_cap) {
     HIPCK(hipStreamSynchronize(e->stream));
@@ -500,16 +562,30 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   if (!e->p.nwork) {
     HIPCK(hipMalloc((void **)&e->p.nwork, 256));
     e->p.nsys = e->p.nwork + 16;
+    e->p.ncellwork = e->p.nwork + 32;
+    e->p.nitems = e->p.nwork + 48;
+    invalidate_graphs(e);
+  }
+  if ((int64_t)e->h_items.size() + 1 > e->items_cap) {
+    HIPCK(hipStreamSynchronize(e->stream));
+    const int64_t nc = std::max<int64_t>(1024, (int64_t)e->h_items.size() * 2 + 1);
+    if (e->p.items) HIPCK(hipFree(e->p.items));
+    HIPCK(hipMalloc((void **)&e->p.items, sizeof(int4) * (size_t)nc));
+    e->items_cap = nc;
     invalidate_graphs(e);
   }
   if (nslots > e->cap.slots) return set_error(e, SOP_ERR_INVALID, "batch larger than slot capacity");
   const int32_t nw = (int32_t)e->h_work.size(), ns = nslots;
   if (nw > 0) HIPCK(hipMemcpy(e->p.work, e->h_work.data(), 4 * (size_t)nw, hipMemcpyHostToDevice));
   if (ns > 0) HIPCK(hipMemcpy(e->p.syslist, e->h_syslist.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
+  const int32_t ni = (int32_t)e->h_items.size();
+  if (ni > 0) HIPCK(hipMemcpy(e->p.items, e->h_items.data(), sizeof(int4) * (size_t)ni, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.nitems, &ni, 4, hipMemcpyHostToDevice));
   e->cur_nwork = nw;
   e->cur_nsys = ns;
+  e->cur_nitems = ni;
   return SOP_OK;
 }
 
@@ -537,7 +613,7 @@ sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslo
 static std::string graph_key(const sop_engine *e, int G, int n_nl, int n_pl) {
   std::ostringstream os;
   os << G << '|' << n_nl << '|' << n_pl << '|' << e->opts.lanes_per_bead << '|' << grid_tiles(e) << '|'
-     << grid_sys(e);
+     << grid_sys(e) << '|' << grid_items(e) << '|' << e->cur_lds;
   return os.str();
 }
 
@@ -555,9 +631,8 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
     const int gt = grid_tiles(e), gs = grid_sys(e), lpb = e->opts.lanes_per_bead;
     HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < G; ++k) {
-      if (k % n_nl == 0) (void)launch_nl(e->p, gt, gs, k, 0, e->stream);
-      if (k % n_pl == 0) (void)launch_pl(e->p, gt, k, 0, e->stream);
-      (void)launch_step(e->p, MODE_STEP, lpb, gt, k, e->stream);
+      if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, gt, gs, k, 0, e->stream);
+      (void)launch_step(e->p, MODE_STEP, lpb, grid_items(e), k, e->cur_lds, e->stream);
     }
     (void)launch_advance(e->p, G, e->stream);
     HIPCK(hipStreamEndCapture(e->stream, &graph));
@@ -589,9 +664,8 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
       any_nl |= ((st[q] + k) % nnl[q]) == 0;
       any_pl |= ((st[q] + k) % npl[q]) == 0;
     }
-    if (any_nl) HIPCK(launch_nl(e->p, gt, gs, (int)k, 0, e->stream));
-    if (any_pl) HIPCK(launch_pl(e->p, gt, (int)k, 0, e->stream));
-    HIPCK(launch_step(e->p, MODE_STEP, lpb, gt, (int)k, e->stream));
+    if (any_nl || any_pl) HIPCK(launch_lists(e->p, gt, gs, (int)k, 0, e->stream));
+    HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), (int)k, e->cur_lds, e->stream));
   }
   HIPCK(launch_advance(e->p, (int)nsteps, e->stream));
   return SOP_OK;
@@ -618,6 +692,7 @@ void sop_engine_opts_default(sop_engine_opts *o) {
   o->lanes_per_bead = 16;
   o->grid_blocks = 0;
   o->use_graphs = 1;
+  o->lds_beads_max = 1024;
   o->pool_bytes = 0;
 }
 
@@ -638,6 +713,7 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     if (e->opts.pl_cap_max <= 0) e->opts.pl_cap_max = def.pl_cap_max;
     if (e->opts.cell_cap_max <= 0) e->opts.cell_cap_max = def.cell_cap_max;
     if (e->opts.lanes_per_bead <= 0) e->opts.lanes_per_bead = def.lanes_per_bead;
+    if (e->opts.lds_beads_max == 0) e->opts.lds_beads_max = def.lds_beads_max;
   }
   const int lpb = e->opts.lanes_per_bead;
   if (lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) {
@@ -655,6 +731,12 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     return SOP_ERR_HIP;
   }
   e->num_cu = prop.multiProcessorCount;
+  // the step kernel keeps 1 KB of static LDS for the tile's forces
+  e->lds_limit = prop.sharedMemPerBlock > 2048 ? prop.sharedMemPerBlock - 2048 : 0;
+  if (step_lds_limit(e->lds_limit) != hipSuccess) {
+    (void)hipGetLastError();
+    e->lds_limit = 65536 - 2048;
+  }
   // A blocking stream: synchronous hipMemcpy calls on the null stream order
   // after the engine's kernels, so host reads never race device writes.
   if (hipStreamCreate(&e->stream) != hipSuccess) {
@@ -716,7 +798,8 @@ void sop_engine_destroy(sop_engine *e) {
   for (auto &a : e->bead_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(bead pool)");
   for (auto &a : e->nat_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(native pool)");
   for (auto &a : e->cell_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(cell pool)");
-  void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.tile_sys, e->p.sys, e->p.sysout, e->p.syslist, e->p.work, e->p.nwork};
+  void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.tile_sys, e->p.sys,      e->p.sysout,
+                    e->p.syslist, e->p.work,    e->p.nwork,    e->p.cellwork, e->p.items};
   for (void *q : others) if (q) note(hipFree(q), "hipFree");
   if (e->stream) note(hipStreamDestroy(e->stream), "hipStreamDestroy");
   if (first != hipSuccess) std::fprintf(stderr, "sop_engine_destroy: %s: %s\n", what, hipGetErrorString(first));
@@ -758,7 +841,7 @@ sop_status sop_update_neighbor_list(sop_engine *e, int32_t sys) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_nl(e->p, grid_tiles(e), grid_sys(e), 0, 1, e->stream));
+  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_NL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -767,7 +850,16 @@ sop_status sop_update_pair_list(sop_engine *e, int32_t sys) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_pl(e->p, grid_tiles(e), 0, 1, e->stream));
+  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_PL, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, &sys, 1);
+}
+
+// The neighbor list is materialized on demand from its snapshot.
+static sop_status materialize_nl(sop_engine *e, int32_t sys) {
+  sop_status st = engine_set_batch(e, &sys, 1);
+  if (st) return st;
+  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, MATERIALIZE_NL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -775,6 +867,10 @@ sop_status sop_update_pair_list(sop_engine *e, int32_t sys) {
 sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, int64_t cap, int64_t *n) {
   if (!valid_sys(e, sys) || !n) return set_error(e, SOP_ERR_INVALID, "bad arguments");
   HIPCK(hipStreamSynchronize(e->stream));
+  if (which == SOP_LIST_NL_NATIVE || which == SOP_LIST_NL_REP) {
+    sop_status st = materialize_nl(e, sys);
+    if (st) return st;
+  }
   const SysHost &h = e->sys[sys];
   const int32_t nb = h.n;
   std::vector<int32_t> cnt((size_t)nb);
@@ -821,6 +917,8 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
 sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *stt) {
   if (!valid_sys(e, sys) || !stt) return set_error(e, SOP_ERR_INVALID, "bad arguments");
   HIPCK(hipStreamSynchronize(e->stream));
+  sop_status st = materialize_nl(e, sys);
+  if (st) return st;
   const SysHost &h = e->sys[sys];
   std::vector<int32_t> c((size_t)h.n);
   int64_t tot[4];
@@ -845,8 +943,8 @@ sop_status sop_force_eval(sop_engine *e, int32_t sys, int32_t with_noise) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, grid_tiles(e), 0,
-                    e->stream));
+  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, grid_items(e), 0,
+                    e->cur_lds, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -1054,9 +1152,9 @@ sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys,
   hipEvent_t a, b;
   HIPCK(hipEventCreate(&a));
   HIPCK(hipEventCreate(&b));
-  const int gt = grid_tiles(e), lpb = e->opts.lanes_per_bead;
+  const int lpb = e->opts.lanes_per_bead;
   HIPCK(hipEventRecord(a, e->stream));
-  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, gt, r, e->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), r, e->cur_lds, e->stream));
   HIPCK(hipEventRecord(b, e->stream));
   HIPCK(hipEventSynchronize(b));
   float ms = 0.0f;

@@ -169,10 +169,68 @@ __device__ __forceinline__ int cell_index(const SysDev &S, float4 x) {
   return (cz * S.dims[1] + cy) * S.dims[0] + cx;
 }
 
-// ------------------------------------------------- neighbor-list rebuild
+// ------------------------------------------------------------ list rebuilds
+//
+// The neighbor list is defined by the snapshot pos_ref taken at its last
+// rebuild: NL = master pairs with d2(pos_ref) < rc_nl^2, and the pair list is
+// PL = NL pairs with d2(x) < rc_pl^2.  When every bead has moved less than
+// skin/2 = (rc_nl - rc_pl)/2 since the snapshot, every pair with
+// d(x) < rc_pl has d(pos_ref) < rc_pl + 2 max|dx| < rc_nl, so PL equals the
+// master pairs with d2(x) < rc_pl^2: it is built straight from a cell grid of
+// edge >= rc_pl (BUILD_DIRECT) and the neighbor list need not be written.
+// Otherwise the neighbor list is materialized from pos_ref and filtered
+// (BUILD_NL).  Both give the same set as the reference's two-level scheme.
+
+__device__ __forceinline__ const float4 *build_src(const DevPtrs &p, const SysDev &S, int kstep) {
+  return S.build == BUILD_DIRECT ? buf_in(p, S.parity + kstep) : p.pos_ref;
+}
+
+// Per system: snapshot and build decision.
+__global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
+  __shared__ float red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
+    const int s = p.syslist[q];
+    SysDev &S = p.sys[s];
+    const bool mat = (flags & MATERIALIZE_NL) != 0;
+    const bool nl = !mat && ((flags & FORCE_NL) || due_nl(S, kstep));
+    const bool pl = !mat && ((flags & FORCE_PL) || due_pl(S, kstep));
+    const float4 *x = buf_in(p, S.parity + kstep);
+    float d2max = 0.0f;
+    for (int b = tid; b < S.n; b += blockDim.x) {
+      const int i = S.bead_off + b;
+      const float4 xi = x[i];
+      if (nl) {
+        p.pos_ref[i] = xi;
+      } else if (pl) {
+        const float4 r = p.pos_ref[i];
+        const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
+        d2max = fmaxf(d2max, dx * dx + dy * dy + dz * dz);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) d2max = fmaxf(d2max, __shfl_xor(d2max, o));
+    if (lane == 0) red[wave] = d2max;
+    __syncthreads();
+    if (tid == 0) {
+      const float dmax = sqrtf(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
+      int build = BUILD_NONE;
+      if (mat) {
+        build = S.nl_valid ? BUILD_NONE : BUILD_NL_ONLY;
+      } else {
+        if (nl) S.nl_valid = 0;
+        if (pl) build = (S.skin > 0.0f && (nl || 2.0f * dmax < S.skin)) ? BUILD_DIRECT : BUILD_NL;
+        else if (flags & FORCE_NL) build = BUILD_NL_ONLY;
+      }
+      S.build = build;
+    }
+    __syncthreads();
+  }
+}
 
 // Per system: bounding box -> open cell grid; zero the cell histogram.
-__global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep, int force_all) {
+__global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
   __shared__ float red[6][4];
   __shared__ int s_ncells;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -180,11 +238,11 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep, int force_al
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
-    if (!(force_all || due_nl(S, kstep))) continue;
-    const float4 *pin = buf_in(p, S.parity + kstep);
+    if (S.build == BUILD_NONE) continue;
+    const float4 *src = build_src(p, S, kstep);
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int b = tid; b < S.n; b += blockDim.x) {
-      const float4 x = pin[S.bead_off + b];
+      const float4 x = src[S.bead_off + b];
       mn[0] = fminf(mn[0], x.x); mx[0] = fmaxf(mx[0], x.x);
       mn[1] = fminf(mn[1], x.y); mx[1] = fmaxf(mx[1], x.y);
       mn[2] = fminf(mn[2], x.z); mx[2] = fmaxf(mx[2], x.z);
@@ -203,6 +261,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep, int force_al
     }
     __syncthreads();
     if (tid == 0) {
+      const float min_edge = S.build == BUILD_DIRECT ? S.cell_min_edge_pl : S.cell_min_edge;
       float ext_max = 0.0f;
       int nc = 1;
       for (int d = 0; d < 3; ++d) {
@@ -210,7 +269,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep, int force_al
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { a = fminf(a, red[d][w]); b = fmaxf(b, red[3 + d][w]); }
         const float ext = b - a;
         int dims = 1;
-        if (ext > 0.0f) dims = (int)floorf(ext / S.cell_min_edge);
+        if (ext > 0.0f) dims = (int)floorf(ext / min_edge);
         dims = min(max(dims, 1), S.dmax);
         S.org[d] = a;
         S.inv_edge[d] = ext > 0.0f ? (float)dims / ext : 0.0f;
@@ -236,16 +295,16 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep, int force_al
 }
 
 // Per bead: cell id and histogram.
-__global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep, int force_all) {
+__global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep) {
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (!(force_all || due_nl(S, kstep))) continue;
+    if (S.build == BUILD_NONE) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
-      const int c = cell_index(S, buf_in(p, S.parity + kstep)[i]);
+      const int c = cell_index(S, build_src(p, S, kstep)[i]);
       p.cell_of[i] = c;
       atomicAdd(&p.cell_cnt[S.cell_base + c], 1);
     }
@@ -253,14 +312,14 @@ __global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep, int for
 }
 
 // Per system: exclusive scan of the histogram (DPP wave scan + wave totals).
-__global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p, int kstep, int force_all) {
+__global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
   __shared__ int wsum[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
-    if (!(force_all || due_nl(S, kstep))) continue;
+    if (S.build == BUILD_NONE) continue;
     const int nc = S.ncells, cb = S.cell_base;
     int carry = 0;
     for (int c0 = 0; c0 < nc; c0 += 256) {
@@ -290,13 +349,13 @@ __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p, int kstep, int for
 }
 
 // Per bead: scatter into its cell's segment (arbitrary in-cell order).
-__global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p, int kstep, int force_all) {
+__global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p) {
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (!(force_all || due_nl(S, kstep))) continue;
+    if (S.build == BUILD_NONE) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = p.cell_of[i];
@@ -307,15 +366,16 @@ __global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p, int kstep, int f
   }
 }
 
-// Per bead: rank among the beads of its cell by index -> sorted segment.
-// Makes the in-cell order, and with it every list, independent of atomics.
-__global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep, int force_all) {
+// Per bead: rank among the beads of its cell by index -> sorted segment, with
+// a cell-ordered copy of the coordinates.  The in-cell order, and with it every
+// list, is independent of the order of the atomics above.
+__global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep) {
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (!(force_all || due_nl(S, kstep))) continue;
+    if (S.build == BUILD_NONE) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = p.cell_of[i];
@@ -324,33 +384,330 @@ __global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep, int forc
       if (st >= 0 && en <= S.n) {
         for (int m = st; m < en; ++m) rank += p.cell_tmp[S.bead_off + m] < i;
         const int pos = st + rank;
-        if (pos < S.n) p.cell_sorted[S.bead_off + pos] = i;
+        if (pos < S.n) {
+          p.cell_sorted[S.bead_off + pos] = i;
+          p.pos_sorted[S.bead_off + pos] = build_src(p, S, kstep)[i];
+        }
       }
     }
   }
 }
 
-// Wave per bead: native neighbor list from the bead's native table, and the
-// non-native neighbor list from the 27 surrounding cells.
-__global__ __launch_bounds__(256) void k_nl_build(DevPtrs p, int kstep, int force_all) {
+__device__ __forceinline__ bool native_pair(const int32_t *nj, int nn, int j) {
+  int lo = 0, hi = nn;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (nj[mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nn && nj[lo] == j;
+}
+
+// Native partners of the current bead, as a wave-private LDS bitmap keyed by
+// (local index mod 65536); a set bit is confirmed by the exact search unless
+// the system is small enough for the key to be the index itself.
+constexpr int NATBITS_WORDS = 2048;
+
+__device__ __forceinline__ void natbits_mark(uint32_t *bits, const int32_t *natj, int nn, int bead_off, bool set) {
+  const int lane = threadIdx.x & 63;
+  for (int k = lane; k < nn; k += 64) {
+    const int jl = (natj[k] - bead_off) & 0xFFFF;
+    if (set) atomicOr(&bits[jl >> 5], 1u << (jl & 31));
+    else bits[jl >> 5] = 0u;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Candidate scan of the 27 cells around xi as 9 rows of 3 contiguous cells,
+// reading the cell-ordered copy of the coordinates.  Non-native master pairs
+// with d2 below the cutoff of their type pair (rcA: tj = 1, rcB: tj = 2) are
+// appended to out[0..cap) with ballot/mbcnt
+// compaction; returns the count (which may exceed cap).
+__device__ __forceinline__ int scan_cells(const DevPtrs &p, const SysDev &S, int i, float4 xi,
+                                          float rcA, float rcB, int ns, int nn, int32_t *out, int cap,
+                                          uint32_t *bits) {
+  const int lane = threadIdx.x & 63;
+  const int wi = __float_as_int(xi.w), ci = wi >> 2;
+  const float boxl = S.boxl;
+  const int cx = min(max((int)floorf((xi.x - S.org[0]) * S.inv_edge[0]), 0), S.dims[0] - 1);
+  const int cy = min(max((int)floorf((xi.y - S.org[1]) * S.inv_edge[1]), 0), S.dims[1] - 1);
+  const int cz = min(max((int)floorf((xi.z - S.org[2]) * S.inv_edge[2]), 0), S.dims[2] - 1);
+  const int x0 = max(cx - 1, 0), x1 = min(cx + 1, S.dims[0] - 1);
+  const int32_t *natj = p.nat_j + ns;
+  const bool exact_key = S.n <= 65536;
+  natbits_mark(bits, natj, nn, S.bead_off, true);
+  int cnt = 0;
+  for (int zz = max(cz - 1, 0); zz <= min(cz + 1, S.dims[2] - 1); ++zz)
+    for (int yy = max(cy - 1, 0); yy <= min(cy + 1, S.dims[1] - 1); ++yy) {
+      const int row = (zz * S.dims[1] + yy) * S.dims[0];
+      const int st = max(p.cell_start[S.cell_base + row + x0], 0);
+      const int en = min(p.cell_start[S.cell_base + row + x1 + 1], S.n);
+      for (int m0 = st; m0 < en; m0 += 64) {
+        const int m = m0 + lane;
+        bool pred = false;
+        int j = 0;
+        if (m < en) {
+          j = p.cell_sorted[S.bead_off + m];
+          const float4 xj = p.pos_sorted[S.bead_off + m];
+          const int wj = __float_as_int(xj.w);
+          const int dj = j - i;
+          const bool excl = (j == i) || (((wj >> 2) == ci) && dj >= -2 && dj <= 2) ||
+                            j < S.bead_off || j >= S.bead_off + S.n;
+          if (!excl && dist2_exact(xi, xj, boxl) < ((wj & 3) == 2 ? rcB : rcA)) {
+            const int jl = (j - S.bead_off) & 0xFFFF;
+            bool nat = (bits[jl >> 5] >> (jl & 31)) & 1u;
+            if (nat && !exact_key) nat = native_pair(natj, nn, j);
+            pred = !nat;
+          }
+        }
+        const uint64_t mk = __ballot(pred);
+        if (pred) {
+          const int o = cnt + lanes_below(mk);
+          if (o < cap) out[o] = j;
+        }
+        cnt += __popcll(mk);
+      }
+    }
+  natbits_mark(bits, natj, nn, S.bead_off, false);
+  return cnt;
+}
+
+// Per batch: exclusive scan of the home-cell counts of the systems that build
+// their pair list directly (one block).
+__global__ __launch_bounds__(256) void k_cellwork(DevPtrs p) {
+  __shared__ int wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsys = *p.nsys;
+  int carry = 0;
+  for (int q0 = 0; q0 < nsys; q0 += 256) {
+    const int q = q0 + tid;
+    int v = 0;
+    if (q < nsys) {
+      const SysDev &S = p.sys[p.syslist[q]];
+      v = S.build == BUILD_DIRECT ? S.ncells : 0;
+    }
+    const int incl = wave_incl_scan(v);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      off += (w < wave) ? wsum[w] : 0;
+      tot += wsum[w];
+    }
+    if (q < nsys) p.cellwork[q] = carry + off + incl - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    p.cellwork[nsys] = carry;
+    *p.ncellwork = carry;
+  }
+}
+
+// Per system of the batch: list its home cells for the direct build.
+__global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
+    const SysDev &S = p.sys[p.syslist[q]];
+    if (S.build != BUILD_DIRECT) continue;
+    const int off = p.cellwork[q], nc = S.ncells;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) p.cellitems[off + c] = (q << 20) | c;
+  }
+}
+
+// Block per home cell: the pair lists of the cell's beads straight from the
+// pair-cutoff grid.  The bounds of the 9 neighbor rows and all their
+// candidates are staged in LDS together (CAND_MAX at a time); every home bead
+// of the cell tests them.  Each bead's list comes out in row order, then
+// cell-sorted order: the order of a per-bead scan of the same cells.
+constexpr int HOME_MAX = 64;   // home beads per pass
+constexpr int HOME_NAT = 32;   // natives staged per home bead (more: exact global search)
+constexpr int CAND_MAX = 1024; // candidates staged per pass
+
+__global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
+  __shared__ float4 s_home[HOME_MAX];
+  __shared__ int s_hj[HOME_MAX], s_hcnt[HOME_MAX], s_hnn[HOME_MAX], s_hns[HOME_MAX];
+  __shared__ int s_nat[HOME_MAX][HOME_NAT];
+  __shared__ uint32_t s_mask[HOME_MAX][8];
+  __shared__ float4 s_cand[CAND_MAX];
+  __shared__ int s_cj[CAND_MAX];
+  __shared__ int s_rst[9], s_roff[10];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int total = *p.ncellwork;
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int item = p.cellitems[g];
+    const int s = p.syslist[item >> 20];
+    const int c = item & 0xFFFFF;
+    const SysDev &S = p.sys[s];
+    const int bo = S.bead_off, cb = S.cell_base, n = S.n;
+    const float boxl = S.boxl;
+    const int pl_cap = S.pl_cap;
+    const int64_t pl_base = S.pl_base;
+    const float rc2_11 = S.rc2_pl_rep[0], rc2_12 = S.rc2_pl_rep[1], rc2_21 = S.rc2_pl_rep[2],
+                rc2_22 = S.rc2_pl_rep[3];
+    const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
+    const int cx = c % d0, cy = (c / d0) % d1, cz = c / (d0 * d1);
+    const int hs = max(p.cell_start[cb + c], 0), he = min(p.cell_start[cb + c + 1], n);
+    if (hs >= he) continue;
+    // row bounds of the 3x3 neighbor rows (x-range cx-1..cx+1)
+    if (tid < 9) {
+      const int zz = cz + tid / 3 - 1, yy = cy + tid % 3 - 1;
+      int st = 0, len = 0;
+      if (zz >= 0 && zz < d2 && yy >= 0 && yy < d1) {
+        const int row = (zz * d1 + yy) * d0;
+        st = max(p.cell_start[cb + row + max(cx - 1, 0)], 0);
+        len = max(min(p.cell_start[cb + row + min(cx + 1, d0 - 1) + 1], n) - st, 0);
+      }
+      s_rst[tid] = st;
+      s_roff[tid + 1] = len;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_roff[0] = 0;
+      for (int r = 1; r <= 9; ++r) s_roff[r] += s_roff[r - 1];
+    }
+    __syncthreads();
+    const int T = s_roff[9];
+    for (int h0 = hs; h0 < he; h0 += HOME_MAX) {
+      const int nh = min(HOME_MAX, he - h0);
+      if (tid < nh) {
+        const int j = p.cell_sorted[bo + h0 + tid];
+        s_home[tid] = p.pos_sorted[bo + h0 + tid];
+        s_hj[tid] = j;
+        s_hcnt[tid] = 0;
+        s_hns[tid] = p.nat_start[j];
+        s_hnn[tid] = p.nat_cnt[j];
+#pragma unroll
+        for (int w = 0; w < 8; ++w) s_mask[tid][w] = 0u;
+      }
+      __syncthreads();
+      for (int h = wave; h < nh; h += 4) {
+        const int ns = s_hns[h], nn = s_hnn[h];
+        for (int k = lane; k < nn; k += 64) {
+          const int jn = p.nat_j[ns + k];
+          if (k < HOME_NAT) s_nat[h][k] = jn;
+          const int key = (jn - bo) & 255;
+          atomicOr(&s_mask[h][key >> 5], 1u << (key & 31));
+        }
+      }
+      for (int b0 = 0; b0 < T; b0 += CAND_MAX) {
+        const int nc = min(CAND_MAX, T - b0);
+        for (int ci = tid; ci < nc; ci += 256) {
+          const int idx = b0 + ci;
+          int r = 0;
+#pragma unroll
+          for (int q = 1; q < 9; ++q) r += s_roff[q] <= idx;
+          const int m = s_rst[r] + (idx - s_roff[r]);
+          s_cand[ci] = p.pos_sorted[bo + m];
+          s_cj[ci] = p.cell_sorted[bo + m];
+        }
+        __syncthreads();
+        for (int h = wave; h < nh; h += 4) {
+          const float4 xi = s_home[h];
+          const int ji = s_hj[h];
+          const int wi = __float_as_int(xi.w), chi = wi >> 2;
+          const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
+          const int nn = s_hnn[h];
+          int cnt = s_hcnt[h];
+          int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
+          for (int k0 = 0; k0 < nc; k0 += 64) {
+            const int ci = k0 + lane;
+            bool pred = false;
+            int j = 0;
+            if (ci < nc) {
+              j = s_cj[ci];
+              const float4 xj = s_cand[ci];
+              const int wj = __float_as_int(xj.w);
+              const int dj = j - ji;
+              const bool excl = (j == ji) || (((wj >> 2) == chi) && dj >= -2 && dj <= 2);
+              if (!excl && dist2_exact(xi, xj, boxl) < ((wj & 3) == 2 ? rcB : rcA)) {
+                const int key = (j - bo) & 255;
+                bool nat = (s_mask[h][key >> 5] >> (key & 31)) & 1u;
+                if (nat) {
+                  if (nn <= HOME_NAT) {
+                    bool hit = false;
+                    for (int q = 0; q < nn; ++q) hit |= s_nat[h][q] == j;
+                    nat = hit;
+                  } else {
+                    nat = native_pair(p.nat_j + s_hns[h], nn, j);
+                  }
+                }
+                pred = !nat;
+              }
+            }
+            const uint64_t mk = __ballot(pred);
+            if (pred) {
+              const int o = cnt + lanes_below(mk);
+              if (o < pl_cap) out[o] = j;
+            }
+            cnt += __popcll(mk);
+          }
+          if (lane == 0) s_hcnt[h] = cnt;
+        }
+        __syncthreads();
+      }
+      // counts, and the native pair lists of the home beads
+      if (tid < nh) {
+        const int j = s_hj[tid], cr = s_hcnt[tid];
+        p.plrep_cnt[j] = min(cr, pl_cap);
+        if (cr > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+      }
+      const float4 *x = buf_in(p, S.parity + kstep);
+      const double plf = S.pl_factor;
+      const float pla = S.pl_abs;
+      for (int h = wave; h < nh; h += 4) {
+        const int i = s_hj[h], ns = s_hns[h], nn = s_hnn[h];
+        const float4 xi = s_home[h];
+        int cnt = 0;
+        for (int k0 = 0; k0 < nn; k0 += 64) {
+          const int k = k0 + lane;
+          bool pred = false;
+          int j = 0;
+          float r0 = 0.0f;
+          if (k < nn) {
+            j = p.nat_j[ns + k];
+            r0 = p.nat_r0[ns + k];
+            const float rc = cutoff_of(plf, pla, r0);
+            pred = dist2_exact(xi, x[j], boxl) < rc * rc;
+          }
+          const uint64_t mm = __ballot(pred);
+          if (pred) {
+            const int o = ns + cnt + lanes_below(mm);
+            p.platt_j[o] = j;
+            p.platt_r0[o] = r0;
+          }
+          cnt += __popcll(mm);
+        }
+        if (lane == 0) p.platt_cnt[i] = cnt;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Wave per bead: materialize the neighbor list of the snapshot pos_ref.
+__global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
+  __shared__ uint32_t s_bits[4][NATBITS_WORDS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = lane; k < NATBITS_WORDS; k += 64) s_bits[wave][k] = 0u;
+  __builtin_amdgcn_wave_barrier();
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
     if (t < 0) continue;
     const int s = p.tile_sys[t];
     const SysDev &S = p.sys[s];
-    if (!(force_all || due_nl(S, kstep))) continue;
-    const float4 *pin = buf_in(p, S.parity + kstep);
-    const float boxl = S.boxl;
+    if (S.build != BUILD_NL && S.build != BUILD_NL_ONLY) continue;
+    const float4 *ref = p.pos_ref;
     const int bend = min(t * TILE + TILE, S.bead_off + S.n);
     for (int q = 0; q < TILE / 4; ++q) {
       const int i = t * TILE + wave * (TILE / 4) + q;
       if (i >= bend) break;
-      const float4 xi = pin[i];
-      const int wi = __float_as_int(xi.w), ti = wi & 3, ci = wi >> 2;
+      const float4 xi = ref[i];
+      const int ti = __float_as_int(xi.w) & 3;
       const int ns = p.nat_start[i], nn = p.nat_cnt[i];
-      // natives
       int cnt = 0;
       for (int k0 = 0; k0 < nn; k0 += 64) {
         const int k = k0 + lane;
@@ -361,7 +718,7 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p, int kstep, int forc
           j = p.nat_j[ns + k];
           r0 = p.nat_r0[ns + k];
           const float rc = cutoff_of(S.nl_factor, S.nl_abs, r0);
-          pred = dist2_exact(xi, pin[j], boxl) < rc * rc;
+          pred = dist2_exact(xi, ref[j], S.boxl) < rc * rc;
         }
         const uint64_t m = __ballot(pred);
         if (pred) {
@@ -372,57 +729,20 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p, int kstep, int forc
         cnt += __popcll(m);
       }
       if (lane == 0) p.nlatt_cnt[i] = cnt;
-      // non-natives
-      const int cx = min(max((int)floorf((xi.x - S.org[0]) * S.inv_edge[0]), 0), S.dims[0] - 1);
-      const int cy = min(max((int)floorf((xi.y - S.org[1]) * S.inv_edge[1]), 0), S.dims[1] - 1);
-      const int cz = min(max((int)floorf((xi.z - S.org[2]) * S.inv_edge[2]), 0), S.dims[2] - 1);
       const int64_t base = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
-      const float *rc2row = &S.rc2_nl_rep[(ti - 1) * 2];
-      int cr = 0;
-      for (int zz = max(cz - 1, 0); zz <= min(cz + 1, S.dims[2] - 1); ++zz)
-        for (int yy = max(cy - 1, 0); yy <= min(cy + 1, S.dims[1] - 1); ++yy)
-          for (int xx = max(cx - 1, 0); xx <= min(cx + 1, S.dims[0] - 1); ++xx) {
-            const int c = (zz * S.dims[1] + yy) * S.dims[0] + xx;
-            const int st = max(p.cell_start[S.cell_base + c], 0);
-            const int en = min(p.cell_start[S.cell_base + c + 1], S.n);
-            for (int m0 = st; m0 < en; m0 += 64) {
-              const int m = m0 + lane;
-              bool pred = false;
-              int j = 0;
-              if (m < en) j = p.cell_sorted[S.bead_off + m];
-              if (m < en && j >= S.bead_off && j < S.bead_off + S.n) {
-                const float4 xj = pin[j];
-                const int wj = __float_as_int(xj.w);
-                const int dj = j - i;
-                const bool excl = (j == i) || (((wj >> 2) == ci) && dj >= -2 && dj <= 2);
-                if (!excl) {
-                  const float d2 = dist2_exact(xi, xj, boxl);
-                  if (d2 < rc2row[(wj & 3) - 1]) {
-                    bool nat = false;
-                    for (int k = 0; k < nn; ++k) nat |= (p.nat_j[ns + k] == j);
-                    pred = !nat;
-                  }
-                }
-              }
-              const uint64_t mk = __ballot(pred);
-              if (pred) {
-                const int o = cr + lanes_below(mk);
-                if (o < S.nl_cap) p.nlrep_j[base + o] = j;
-              }
-              cr += __popcll(mk);
-            }
-          }
+      const float rcA = S.rc2_nl_rep[(ti - 1) * 2], rcB = S.rc2_nl_rep[(ti - 1) * 2 + 1];
+      const int cr = scan_cells(p, S, i, xi, rcA, rcB, ns, nn, p.nlrep_j + base, S.nl_cap, s_bits[wave]);
       if (lane == 0) {
         p.nlrep_cnt[i] = min(cr, S.nl_cap);
         if (cr > S.nl_cap) atomicOr(&p.sys[s].err, ERR_NL_OVF);
+        p.sys[s].nl_valid = 1;
       }
     }
   }
 }
 
-// ------------------------------------------------------ pair-list filter
-
-__global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep, int force_all) {
+// Wave per bead: pair list = materialized neighbor list filtered at x.
+__global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -430,7 +750,7 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep, int forc
     if (t < 0) continue;
     const int s = p.tile_sys[t];
     const SysDev &S = p.sys[s];
-    if (!(force_all || due_pl(S, kstep))) continue;
+    if (S.build != BUILD_NL) continue;
     const float4 *pin = buf_in(p, S.parity + kstep);
     const float boxl = S.boxl;
     const int bend = min(t * TILE + TILE, S.bead_off + S.n);
@@ -465,7 +785,7 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep, int forc
       const int64_t nb = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
       const int64_t pb = S.pl_base + (int64_t)(i - S.bead_off) * S.pl_cap;
       const int nr = p.nlrep_cnt[i];
-      const float *rc2row = &S.rc2_pl_rep[(ti - 1) * 2];
+      const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
       int cr = 0;
       for (int m0 = 0; m0 < nr; m0 += 64) {
         const int m = m0 + lane;
@@ -474,7 +794,7 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep, int forc
         if (m < nr) {
           j = p.nlrep_j[nb + m];
           const float4 xj = pin[j];
-          pred = dist2_exact(xi, xj, boxl) < rc2row[w_type(xj.w) - 1];
+          pred = dist2_exact(xi, xj, boxl) < (w_type(xj.w) == 2 ? rcB : rcA);
         }
         const uint64_t mk = __ballot(pred);
         if (pred) {
@@ -493,168 +813,205 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep, int forc
 
 // ----------------------------------------------- fused force + integrate
 
-// LPB lanes gather one bead's partners; a 256-thread block covers one tile of
-// 64 beads in LPB/4 rounds, stages the 64 reduced forces in LDS, and wave 0
-// then draws the random force and moves all 64 beads (one bead per lane).
+// Partner coordinates come from the system's LDS copy (LDS = true) or from
+// global memory.
+template <bool LDS>
+__device__ __forceinline__ float4 partner(const float4 *__restrict__ pin, const float4 *s_pos, int bo, int j) {
+  return LDS ? s_pos[j - bo] : pin[j];
+}
+
+// One tile (64 beads) of a system: LPB lanes gather one bead's partners and
+// reduce; a 256-thread block covers the tile in LPB/4 rounds, stages the 64
+// forces in LDS, and wave 0 draws the random force and moves the 64 beads
+// (one bead per lane).
+template <int LPB, int MODE, bool LDS>
+__device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int s, int t, int kstep,
+                                          const float4 *__restrict__ pin, float4 *__restrict__ pout,
+                                          float4 *sF, const float4 *s_pos) {
+  constexpr int GPW = 64 / LPB;            // bead groups per wave
+  constexpr int ROUNDS = TILE / (4 * GPW); // rounds to cover the tile
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane / LPB, l = lane % LPB;
+  const int bead_off = S.bead_off, n = S.n;
+  const int b0 = t * TILE;
+  const int bend = min(b0 + TILE, bead_off + n);
+  const float boxl = S.boxl, inv_boxl = S.inv_boxl;
+  const bool wrapf = S.wrap_free != 0 || boxl <= 0.0f;
+  const int64_t step = S.step + kstep;
+  for (int r = 0; r < ROUNDS; ++r) {
+    const int lb = r * (4 * GPW) + wave * GPW + g;
+    const int i = b0 + lb;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (i < bend) {
+      const float4 xi = partner<LDS>(pin, s_pos, bead_off, i);
+      const int wi = __float_as_int(xi.w);
+      const int row = ((wi & 3) - 1) * 2;
+      // chain partners i-2, i-1, i+1, i+2: FENE bonds and i,i+2 soft sphere
+      if (l < 4) {
+        const int off = l < 2 ? l - 2 : l - 1;
+        const int j = i + off;
+        if (j >= bead_off && j < bead_off + n) {
+          const float4 xj = partner<LDS>(pin, s_pos, bead_off, j);
+          if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
+            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            if (!wrapf) {
+              dx = mi_fast(dx, boxl, inv_boxl);
+              dy = mi_fast(dy, boxl, inv_boxl);
+              dz = mi_fast(dz, boxl, inv_boxl);
+            }
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float c;
+            if (off == 1 || off == -1) {
+              const float r0 = p.bond_r0[off < 0 ? j : i];
+              const float d = sqrtf(d2);
+              const float dev = d - r0;
+              const float gg = 1.0f - dev * dev * S.inv_R0sq;
+              c = S.k_bnd * dev * rcpf(d * gg);
+              if (gg <= 0.0f) atomicOr(&p.sys[s].err, ERR_BOND);
+            } else {
+              const float ir2 = rcpf(d2);
+              const float ir4 = ir2 * ir2;
+              c = -S.ss_f6 * (ir4 * ir4);
+            }
+            fx = fmaf(c, dx, fx);
+            fy = fmaf(c, dy, fy);
+            fz = fmaf(c, dz, fz);
+          }
+        }
+      }
+      // native LJ over the native pair list
+      {
+        const int ns = p.nat_start[i], nc = p.platt_cnt[i];
+        const float eA = S.eps_att[row], eB = S.eps_att[row + 1];
+        for (int k = l; k < nc; k += LPB) {
+          const int j = p.platt_j[ns + k];
+          const float r0 = p.platt_r0[ns + k];
+          const float4 xj = partner<LDS>(pin, s_pos, bead_off, j);
+          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+          if (!wrapf) {
+            dx = mi_fast(dx, boxl, inv_boxl);
+            dy = mi_fast(dy, boxl, inv_boxl);
+            dz = mi_fast(dz, boxl, inv_boxl);
+          }
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const float eps = w_type(xj.w) == 2 ? eB : eA;
+          const float r02 = r0 * r0;
+          const float ir2 = rcpf(d2);
+          const float q2 = r02 * ir2;
+          const float q6 = q2 * q2 * q2;
+          float c = -12.0f * eps * ir2 * q6 * (q6 - 1.0f);
+          c = d2 > 1.0e-7f * r02 ? c : 0.0f;
+          fx = fmaf(c, dx, fx);
+          fy = fmaf(c, dy, fy);
+          fz = fmaf(c, dz, fz);
+        }
+      }
+      // non-native repulsion over the non-native pair list; four entries per
+      // lane per pass keep four independent gathers in flight
+      {
+        const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
+        const int nr = p.plrep_cnt[i];
+        const float s2A = S.sig2_rep[row], s2B = S.sig2_rep[row + 1];
+        const float m12A = S.rep_m12[row], m12B = S.rep_m12[row + 1];
+        const float m6A = S.rep_m6[row], m6B = S.rep_m6[row + 1];
+        const float gA = S.rep_guard[row], gB = S.rep_guard[row + 1];
+        const int32_t *__restrict__ lst = p.plrep_j + pb;
+        auto rep = [&](float4 xj) {
+          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+          if (!wrapf) {
+            dx = mi_fast(dx, boxl, inv_boxl);
+            dy = mi_fast(dy, boxl, inv_boxl);
+            dz = mi_fast(dz, boxl, inv_boxl);
+          }
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const bool tb = w_type(xj.w) == 2;
+          const float s2 = tb ? s2B : s2A;
+          const float ir2 = rcpf(d2);
+          const float q2 = s2 * ir2;
+          const float q6 = q2 * q2 * q2;
+          float c = ir2 * q6 * fmaf(tb ? m12B : m12A, q6, tb ? m6B : m6A);
+          c = d2 > (tb ? gB : gA) ? c : 0.0f;
+          fx = fmaf(c, dx, fx);
+          fy = fmaf(c, dy, fy);
+          fz = fmaf(c, dz, fz);
+        };
+        int k = l;
+        for (; k + 3 * LPB < nr; k += 4 * LPB) {
+          const int j0 = lst[k], j1 = lst[k + LPB], j2 = lst[k + 2 * LPB], j3 = lst[k + 3 * LPB];
+          const float4 x0 = partner<LDS>(pin, s_pos, bead_off, j0), x1 = partner<LDS>(pin, s_pos, bead_off, j1);
+          const float4 x2 = partner<LDS>(pin, s_pos, bead_off, j2), x3 = partner<LDS>(pin, s_pos, bead_off, j3);
+          rep(x0);
+          rep(x1);
+          rep(x2);
+          rep(x3);
+        }
+        for (; k < nr; k += LPB) rep(partner<LDS>(pin, s_pos, bead_off, lst[k]));
+      }
+    }
+    fx = group_sum<LPB>(fx);
+    fy = group_sum<LPB>(fy);
+    fz = group_sum<LPB>(fz);
+    if (l == 0 && i < bend) sF[lb] = make_float4(fx, fy, fz, 0.0f);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const int i = b0 + lane;
+    if (i < bend) {
+      const float4 f = sF[lane];
+      if (MODE == MODE_FORCE) {
+        p.fout[i] = f;
+      } else {
+        float n0, n1, n2;
+        normals3(S.seed_lo, S.seed_hi, (uint32_t)(i - bead_off), step, n0, n1, n2);
+        const float var = S.var;
+        const float ftx = f.x + n0 * var, fty = f.y + n1 * var, ftz = f.z + n2 * var;
+        if (MODE == MODE_FORCE_NOISE) {
+          p.fout[i] = make_float4(ftx, fty, ftz, 0.0f);
+        } else {
+          float4 x = partner<LDS>(pin, s_pos, bead_off, i);
+          const float a5 = S.a5;
+          x.x = x.x + a5 * ftx;
+          x.y = x.y + a5 * fty;
+          x.z = x.z + a5 * ftz;
+          pout[i] = x;
+          if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Fused step over work items {system, first tile, tiles, flags}.  Items with
+// ITEM_LDS first copy their whole system's coordinates into dynamic LDS, so
+// every partner gather is an LDS read; the result is identical either way.
 template <int LPB, int MODE>
 __global__ __launch_bounds__(256) void k_step(DevPtrs p, int kstep) {
   __shared__ float4 sF[TILE];
-  constexpr int GPW = 64 / LPB;              // bead groups per wave
-  constexpr int ROUNDS = TILE / (4 * GPW);   // rounds to cover the tile
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int g = lane / LPB, l = lane % LPB;
-  const int nwork = *p.nwork;
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const int t = p.work[w];
-    if (t < 0) continue;
-    const int s = p.tile_sys[t];
+  extern __shared__ float4 s_pos[];
+  const int tid = threadIdx.x;
+  const int nitems = *p.nitems;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 it = p.items[w];
+    if (it.x < 0) continue;
+    const int s = it.x;
     const SysDev &S = p.sys[s];
-    const int64_t step = S.step + kstep;
-    const int bead_off = S.bead_off, n = S.n;
-    const int b0 = t * TILE;
-    const int bend = min(b0 + TILE, bead_off + n);
     const int par = S.parity + kstep;
     const float4 *__restrict__ pin = buf_in(p, par);
     float4 *__restrict__ pout = buf_out(p, par);
-    if (MODE == MODE_STEP && step >= S.step_end) {
-      // finished inside this block: carry the coordinates to the other buffer
-      if (tid < TILE && b0 + tid < bend) pout[b0 + tid] = pin[b0 + tid];
+    if (MODE == MODE_STEP && S.step + kstep >= S.step_end) {
+      // finished inside this block of steps: carry the coordinates over
+      const int a = it.y * TILE, b = min((it.y + it.z) * TILE, S.bead_off + S.n);
+      for (int i = a + tid; i < b; i += blockDim.x) pout[i] = pin[i];
       continue;
     }
-    const float boxl = S.boxl, inv_boxl = S.inv_boxl;
-    const bool wrapf = S.wrap_free != 0 || boxl <= 0.0f;
-    for (int r = 0; r < ROUNDS; ++r) {
-      const int lb = r * (4 * GPW) + wave * GPW + g;
-      const int i = b0 + lb;
-      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-      if (i < bend) {
-        const float4 xi = pin[i];
-        const int wi = __float_as_int(xi.w);
-        const int row = ((wi & 3) - 1) * 2;
-        // chain partners i-2, i-1, i+1, i+2: FENE bonds and i,i+2 soft sphere
-        if (l < 4) {
-          const int off = l < 2 ? l - 2 : l - 1;
-          const int j = i + off;
-          if (j >= bead_off && j < bead_off + n) {
-            const float4 xj = pin[j];
-            if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
-              float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-              if (!wrapf) {
-                dx = mi_fast(dx, boxl, inv_boxl);
-                dy = mi_fast(dy, boxl, inv_boxl);
-                dz = mi_fast(dz, boxl, inv_boxl);
-              }
-              const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              float c;
-              if (off == 1 || off == -1) {
-                const float r0 = p.bond_r0[off < 0 ? j : i];
-                const float d = sqrtf(d2);
-                const float dev = d - r0;
-                const float gg = 1.0f - dev * dev * S.inv_R0sq;
-                c = S.k_bnd * dev * rcpf(d * gg);
-                if (gg <= 0.0f) atomicOr(&p.sys[s].err, ERR_BOND);
-              } else {
-                const float ir2 = rcpf(d2);
-                const float ir4 = ir2 * ir2;
-                c = -S.ss_f6 * (ir4 * ir4);
-              }
-              fx = fmaf(c, dx, fx);
-              fy = fmaf(c, dy, fy);
-              fz = fmaf(c, dz, fz);
-            }
-          }
-        }
-        // native LJ over the native pair list
-        {
-          const int ns = p.nat_start[i], nc = p.platt_cnt[i];
-          const float eA = S.eps_att[row], eB = S.eps_att[row + 1];
-          for (int k = l; k < nc; k += LPB) {
-            const int j = p.platt_j[ns + k];
-            const float r0 = p.platt_r0[ns + k];
-            const float4 xj = pin[j];
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            if (!wrapf) {
-              dx = mi_fast(dx, boxl, inv_boxl);
-              dy = mi_fast(dy, boxl, inv_boxl);
-              dz = mi_fast(dz, boxl, inv_boxl);
-            }
-            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const float eps = w_type(xj.w) == 2 ? eB : eA;
-            const float r02 = r0 * r0;
-            const float ir2 = rcpf(d2);
-            const float q2 = r02 * ir2;
-            const float q6 = q2 * q2 * q2;
-            float c = -12.0f * eps * ir2 * q6 * (q6 - 1.0f);
-            c = d2 > 1.0e-7f * r02 ? c : 0.0f;
-            fx = fmaf(c, dx, fx);
-            fy = fmaf(c, dy, fy);
-            fz = fmaf(c, dz, fz);
-          }
-        }
-        // non-native repulsion over the non-native pair list
-        {
-          const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
-          const int nr = p.plrep_cnt[i];
-          const float s2A = S.sig2_rep[row], s2B = S.sig2_rep[row + 1];
-          const float m12A = S.rep_m12[row], m12B = S.rep_m12[row + 1];
-          const float m6A = S.rep_m6[row], m6B = S.rep_m6[row + 1];
-          const float gA = S.rep_guard[row], gB = S.rep_guard[row + 1];
-          const int32_t *__restrict__ lst = p.plrep_j + pb;
-          for (int k = l; k < nr; k += LPB) {
-            const int j = lst[k];
-            const float4 xj = pin[j];
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            if (!wrapf) {
-              dx = mi_fast(dx, boxl, inv_boxl);
-              dy = mi_fast(dy, boxl, inv_boxl);
-              dz = mi_fast(dz, boxl, inv_boxl);
-            }
-            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const bool tb = w_type(xj.w) == 2;
-            const float s2 = tb ? s2B : s2A;
-            const float ir2 = rcpf(d2);
-            const float q2 = s2 * ir2;
-            const float q6 = q2 * q2 * q2;
-            float c = ir2 * q6 * fmaf(tb ? m12B : m12A, q6, tb ? m6B : m6A);
-            c = d2 > (tb ? gB : gA) ? c : 0.0f;
-            fx = fmaf(c, dx, fx);
-            fy = fmaf(c, dy, fy);
-            fz = fmaf(c, dz, fz);
-          }
-        }
-      }
-      fx = group_sum<LPB>(fx);
-      fy = group_sum<LPB>(fy);
-      fz = group_sum<LPB>(fz);
-      if (l == 0 && i < bend) sF[lb] = make_float4(fx, fy, fz, 0.0f);
+    if (it.w & ITEM_LDS) {
+      for (int b = tid; b < S.n; b += blockDim.x) s_pos[b] = pin[S.bead_off + b];
+      __syncthreads();
+      for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, true>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+    } else {
+      for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, false>(p, S, s, t, kstep, pin, pout, sF, s_pos);
     }
-    __syncthreads();
-    if (wave == 0) {
-      const int i = b0 + lane;
-      if (i < bend) {
-        const float4 f = sF[lane];
-        if (MODE == MODE_FORCE) {
-          p.fout[i] = f;
-        } else {
-          float n0, n1, n2;
-          normals3(S.seed_lo, S.seed_hi, (uint32_t)(i - bead_off), step, n0, n1, n2);
-          const float var = S.var;
-          const float ftx = f.x + n0 * var, fty = f.y + n1 * var, ftz = f.z + n2 * var;
-          if (MODE == MODE_FORCE_NOISE) {
-            p.fout[i] = make_float4(ftx, fty, ftz, 0.0f);
-          } else {
-            float4 x = pin[i];
-            const float a5 = S.a5;
-            x.x = x.x + a5 * ftx;
-            x.y = x.y + a5 * fty;
-            x.z = x.z + a5 * ftz;
-            pout[i] = x;
-            if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
-          }
-        }
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -729,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
             dz = mi_fast(dz, boxl, inv_boxl);
           }
           const float d2 = dx * dx + dy * dy + dz * dz;
-          const float eps = S.eps_att[row + w_type(xj.w) - 1];
+          const float eps = w_type(xj.w) == 2 ? S.eps_att[row + 1] : S.eps_att[row];
           const float r02 = r0 * r0;
           const float q2 = r02 / d2;
           const float q6 = q2 * q2 * q2;
@@ -757,12 +1114,14 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
             dz = mi_fast(dz, boxl, inv_boxl);
           }
           const float d2 = dx * dx + dy * dy + dz * dz;
-          const int tt = row + w_type(xj.w) - 1;
-          const float q2 = S.sig2_rep[tt] / d2;
+          const bool tb = w_type(xj.w) == 2;
+          const float q2 = (tb ? S.sig2_rep[row + 1] : S.sig2_rep[row]) / d2;
           const float q6 = q2 * q2 * q2;
-          er += 0.5 * (double)(S.eps_rep[tt] * (S.c12 * q6 * q6 + S.c6 * q6));
-          if (d2 > S.rep_guard[tt])
-            vir += 0.5 * (double)(-(S.rep_m12[tt] * q6 + S.rep_m6[tt]) * q6);
+          const float ep = tb ? S.eps_rep[row + 1] : S.eps_rep[row];
+          er += 0.5 * (double)(ep * (S.c12 * q6 * q6 + S.c6 * q6));
+          if (d2 > (tb ? S.rep_guard[row + 1] : S.rep_guard[row]))
+            vir += 0.5 * (double)(-((tb ? S.rep_m12[row + 1] : S.rep_m12[row]) * q6 +
+                                    (tb ? S.rep_m6[row + 1] : S.rep_m6[row])) * q6);
         }
       }
 #pragma unroll
@@ -892,37 +1251,53 @@ __global__ void k_selftest(int *out, float *fout) {
 
 #define SOPK_LAUNCH_CHECK() hipGetLastError()
 
-hipError_t launch_nl(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int force_all,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep, force_all);
-  hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, st, p, kstep, force_all);
-  hipLaunchKernelGGL(k_cell_scan, dim3(grid_sys), dim3(256), 0, st, p, kstep, force_all);
-  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_tiles), dim3(64), 0, st, p, kstep, force_all);
-  hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, st, p, kstep, force_all);
-  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep, force_all);
-  return SOPK_LAUNCH_CHECK();
-}
-
-hipError_t launch_pl(const DevPtrs &p, int grid_tiles, int kstep, int force_all, hipStream_t st) {
-  hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep, force_all);
+// One list-rebuild point for the current batch: snapshot / decision, cell
+// grid and counting sort, then whichever of the direct pair-list build or the
+// neighbor-list materialization + filter each system needs.
+hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int flags, hipStream_t st) {
+  hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(256), 0, st, p, kstep, flags);
+  hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_cell_scan, dim3(grid_sys), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_tiles), dim3(64), 0, st, p);
+  hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_cellwork, dim3(1), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   return SOPK_LAUNCH_CHECK();
 }
 
 template <int MODE>
-static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid_tiles, int kstep, hipStream_t st) {
+static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid, int kstep, size_t lds, hipStream_t st) {
   switch (lpb) {
-    case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
-    case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
-    case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
-    default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid_tiles), dim3(256), 0, st, p, kstep); break;
+    case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
+    case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
+    case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
+    default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
   }
   return SOPK_LAUNCH_CHECK();
 }
 
-hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid_tiles, int kstep, hipStream_t st) {
-  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, grid_tiles, kstep, st);
-  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, grid_tiles, kstep, st);
-  return launch_step_mode<MODE_STEP>(p, lpb, grid_tiles, kstep, st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int kstep, size_t lds, hipStream_t st) {
+  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, grid, kstep, lds, st);
+  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, grid, kstep, lds, st);
+  return launch_step_mode<MODE_STEP>(p, lpb, grid, kstep, lds, st);
+}
+
+// Allow the step kernels dynamic LDS up to `bytes`.
+hipError_t step_lds_limit(size_t bytes) {
+  hipError_t h = hipSuccess;
+#define SOPK_LDS_ATTR(L, M)                                                                                      \
+  if (h == hipSuccess)                                                                                           \
+    h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step<L, M>), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            (int)bytes);
+  SOPK_LDS_ATTR(4, 0) SOPK_LDS_ATTR(8, 0) SOPK_LDS_ATTR(16, 0) SOPK_LDS_ATTR(32, 0)
+  SOPK_LDS_ATTR(4, 1) SOPK_LDS_ATTR(8, 1) SOPK_LDS_ATTR(16, 1) SOPK_LDS_ATTR(32, 1)
+  SOPK_LDS_ATTR(4, 2) SOPK_LDS_ATTR(8, 2) SOPK_LDS_ATTR(16, 2) SOPK_LDS_ATTR(32, 2)
+#undef SOPK_LDS_ATTR
+  return h;
 }
 
 hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st) {

@@ -53,7 +53,9 @@ struct SysDev {
   float rc2_nl_rep[4], rc2_pl_rep[4];
   double nl_factor, pl_factor;
   float nl_abs, pl_abs;
-  float cell_min_edge; // >= largest neighbor-list cutoff, with a 0.1% margin
+  float cell_min_edge;    // >= largest neighbor-list cutoff, with a 0.1% margin
+  float cell_min_edge_pl; // >= largest non-native pair-list cutoff, with a 0.1% margin
+  float skin;             // smallest (rc_nl - rc_pl) over pair kinds, less a 1e-5 margin
   float wrap_margin;
   // cell grid, written by k_bbox at each neighbor-list rebuild
   float org[3];
@@ -61,14 +63,31 @@ struct SysDev {
   int32_t dims[3];
   int32_t ncells;
   int32_t wrap_free; // 1: every pair separation is below boxl/2 (minimum image is the identity)
+  // list state, device-written.  The neighbor list is defined by the snapshot
+  // pos_ref taken at its last rebuild; it is materialized only when needed.
+  int32_t build;     // this rebuild point: BUILD_NONE / BUILD_DIRECT / BUILD_NL / BUILD_NL_ONLY
+  int32_t nl_valid;  // 1: the materialized neighbor list matches pos_ref
+  int32_t type_mask; // bit t-1 set when some bead has type t
   // status, device-written
   int32_t err;
-  int32_t pad_[2];
 };
+
+// SysDev::build
+constexpr int BUILD_NONE = 0;    // no list work at this step
+constexpr int BUILD_DIRECT = 1;  // pair list straight from a pair-cutoff cell grid
+constexpr int BUILD_NL = 2;      // materialize the neighbor list from pos_ref, then filter
+constexpr int BUILD_NL_ONLY = 3; // materialize the neighbor list only
+
+// k_skin flags
+constexpr int FORCE_NL = 1;      // rebuild the neighbor list now (snapshot)
+constexpr int FORCE_PL = 2;      // rebuild the pair list now
+constexpr int MATERIALIZE_NL = 4;// materialize the neighbor list of the current snapshot
 
 struct DevPtrs {
   SysDev *sys;
   float4 *pos0, *pos1;  // double-buffered unwrapped coordinates; w = bits (chain<<2 | type)
+  float4 *pos_ref;      // coordinates at the last neighbor-list rebuild
+  float4 *pos_sorted;   // coordinates in cell order (copy made by the cell sort)
   float *bond_r0;       // r0 of bond (i, i+1)
   int32_t *nat_start, *nat_cnt;   // native CSR per bead (two-sided, sorted by partner)
   int32_t *nat_j;       // partner (arena index)
@@ -87,7 +106,15 @@ struct DevPtrs {
   int32_t *nwork;
   int32_t *syslist;     // system slots of the current batch
   int32_t *nsys;
+  int32_t *cellwork;    // per batch position: first home cell of the direct pair-list build
+  int32_t *ncellwork;   // total home cells of the direct pair-list build
+  int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
+  int4 *items;          // step-kernel work: {system slot, first tile, tiles, flags}
+  int32_t *nitems;
 };
+
+constexpr int ITEM_LDS = 1;       // stage the system's coordinates in LDS
+constexpr int ITEM_TILES_MAX = 8; // tiles per step-kernel item
 
 constexpr int EPART_N = 6;   // bond, angle, native, rep, virial, contacts
 constexpr int SYSOUT_N = 16; // e[0..5], contacts, rgsq, sum x,y,z, n

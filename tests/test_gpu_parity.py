@@ -71,7 +71,23 @@ def c2():
     return S.make_chain_system(2, 10000)
 
 
-CASES = {"c1": c1, "multi": multi_chain, "refbox": reference_box, "c2": c2}
+def tight_skin():
+    """Neighbor cutoff 0.05 A above the pair cutoff: beads leave the skin
+    within a few steps, so pair lists come from the materialized neighbor list."""
+    s = S.make_chain_system(31, 400)
+    s.params.nl_abs, s.params.pl_abs = 16.05, 16.0
+    return s
+
+
+def inverted_cutoffs():
+    """Neighbor cutoff below the pair cutoff: PL = NL(14 A, snapshot) & (16 A, now)."""
+    s = S.make_chain_system(32, 300)
+    s.params.nl_abs, s.params.pl_abs = 14.0, 16.0
+    return s
+
+
+CASES = {"c1": c1, "multi": multi_chain, "refbox": reference_box, "c2": c2, "tight": tight_skin,
+         "inverted": inverted_cutoffs}
 
 
 @pytest.fixture(scope="module")
@@ -145,7 +161,8 @@ def test_random_force_is_var_times_philox_normals(eng):
         eng.remove(sid)
 
 
-@pytest.mark.parametrize("case,steps", [("c1", 1000), ("multi", 1000), ("refbox", 300), ("c2", 1000)])
+@pytest.mark.parametrize("case,steps", [("c1", 1000), ("multi", 1000), ("refbox", 300), ("c2", 1000),
+                                        ("tight", 200), ("inverted", 200)])
 def test_trajectory_matches_oracle(eng, case, steps):
     s = CASES[case]()
     sid = eng.add(s)
@@ -157,8 +174,6 @@ def test_trajectory_matches_oracle(eng, case, steps):
         xr = Oracle(s).run(s.xyz, 0, steps, s.params.seed, a5, var, s.params.n_nl, s.params.n_pl)
         rmsd = float(np.sqrt(((x - xr) ** 2).sum(1).mean()))
         assert rmsd <= 1e-4, rmsd
-        # lists after the run still match the oracle on the current coordinates
-        # at the last rebuild point
     finally:
         eng.remove(sid)
 
