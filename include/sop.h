@@ -217,13 +217,18 @@ typedef struct sop_job_result {
 } sop_job_result;
 
 sop_status sop_serve_start(sop_engine *e, const sop_serve_opts *o);
-/* Copies the description; the job runs nsteps from its step0. */
+/* Copies the description; the job runs nsteps from its step0.  All jobs of a
+ * serve session share n_nl and n_pl, and step0 is a multiple of the block.
+ * Called from any thread; returns a status code only (sop_last_error is the
+ * serve thread's). */
 sop_status sop_serve_submit(sop_engine *e, const sop_system_desc *d, int64_t nsteps,
                             int64_t *job);
 /* Non-blocking; when done, xyz (3n floats, may be NULL) receives the final
  * unwrapped coordinates. SOP_ERR_BUSY while the job is queued or running. */
 sop_status sop_serve_poll(sop_engine *e, int64_t job, sop_job_result *r, float *xyz);
 sop_status sop_serve_stop(sop_engine *e);
+/* Device bytes a job holds while resident (the admission budget's unit). */
+int64_t sop_serve_job_bytes(const sop_engine *e, const sop_system_desc *d);
 double sop_clock_seconds(void);
 
 /* --- host-only utilities (no device needed) ------------------------------ */

@@ -126,6 +126,7 @@ SIGNATURES = {
     "sop_serve_submit": (i32, [vp, P(SystemDesc), i64, P(i64)]),
     "sop_serve_poll": (i32, [vp, i64, P(JobResult), P(f32)]),
     "sop_serve_stop": (i32, [vp]),
+    "sop_serve_job_bytes": (i64, [vp, P(SystemDesc)]),
     "sop_clock_seconds": (f64, []),
     "sop_gen_opts_default": (None, [P(GenOpts)]),
     "sop_generate_chain": (i32, [u64, i32, P(GenOpts), P(f32), P(f32)]),

@@ -77,15 +77,20 @@ sop_status hip_check(sop_engine *e, hipError_t h, const char *what) {
 int grid_tiles(const sop_engine *e) {
   int g = e->opts.grid_blocks > 0 ? e->opts.grid_blocks : 8 * e->num_cu;
   g = std::max(8, g / 8 * 8);
+  if (e->fixed_grids) return g;
   const int need = (e->cur_nwork + 7) / 8 * 8;
   return std::max(8, std::min(g, need));
 }
 
-int grid_sys(const sop_engine *e) { return std::max(1, std::min(e->cur_nsys, 2048)); }
+int grid_sys(const sop_engine *e) {
+  if (e->fixed_grids) return 1024;
+  return std::max(1, std::min(e->cur_nsys, 2048));
+}
 
 int grid_items(const sop_engine *e) {
   int g = e->opts.grid_blocks > 0 ? e->opts.grid_blocks : 8 * e->num_cu;
   g = std::max(8, g / 8 * 8);
+  if (e->fixed_grids) return g;
   const int need = (e->cur_nitems + 7) / 8 * 8;
   return std::max(8, std::min(g, need));
 }
@@ -549,6 +554,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   e->h_items.assign(imx * NX, make_int4(-1, 0, 0, 0));
   for (int x = 0; x < NX; ++x)
     for (size_t m = 0; m < ibins[x].size(); ++m) e->h_items[m * NX + x] = ibins[x][m];
+  if (e->fixed_grids && lds_max > 0) lds = std::min((size_t)lds_max * 16, e->lds_limit);
   e->cur_lds = lds;
   e->h_syslist.assign(slots, slots + nslots);
   if ((int64_t)e->h_work.size() + 2 > e->work_cap) {

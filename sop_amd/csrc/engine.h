@@ -87,6 +87,8 @@ struct sop_engine {
   int32_t cur_nwork = 0, cur_nsys = 0, cur_nitems = 0;
   size_t cur_lds = 0;     // dynamic LDS of the step kernel for the current batch
   size_t lds_limit = 0;   // largest dynamic LDS the step kernel may use
+  bool fixed_grids = false; // serving: launch shapes independent of the batch, so
+                            // one captured graph serves every batch composition
 
   // graphs keyed by block shape
   std::map<std::string, hipGraphExec_t> graphs;

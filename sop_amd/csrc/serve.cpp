@@ -1,11 +1,292 @@
-// serve.cpp -- continuous-batching serve loop (see sop.h, "serving").
+// serve.cpp -- continuous-batching request loop.
+//
+// Jobs are submitted from any thread and queued; one serve thread owns the
+// device.  Each iteration it
+//   1. admits queued jobs in arrival order while the HBM budget (job state
+//      bytes) and the job slots allow: upload, initial lists, step_end set;
+//   2. runs one hipGraph block of G steps over every resident job (a job that
+//      reaches its step count inside the block stops there: its coordinates
+//      are carried unchanged through the rest of the block);
+//   3. retires finished jobs: FP64 energies and coordinates are copied back,
+//      the job's device memory is freed and its latency stamps are set.
+// Jobs therefore join and leave the running batch at block boundaries while
+// others keep stepping: continuous batching, not a queue served one by one.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+
 #include "engine.h"
 
-extern "C" {
-sop_status sop_serve_start(sop_engine *e, const sop_serve_opts *o) { (void)o; return e ? SOP_ERR_STATE : SOP_ERR_INVALID; }
-sop_status sop_serve_submit(sop_engine *e, const sop_system_desc *d, int64_t nsteps, int64_t *job) {
-  (void)d; (void)nsteps; (void)job; return e ? SOP_ERR_STATE : SOP_ERR_INVALID; }
-sop_status sop_serve_poll(sop_engine *e, int64_t job, sop_job_result *r, float *xyz) {
-  (void)job; (void)r; (void)xyz; return e ? SOP_ERR_STATE : SOP_ERR_INVALID; }
-sop_status sop_serve_stop(sop_engine *e) { return e ? SOP_OK : SOP_ERR_INVALID; }
+using namespace sopk;
+
+struct sop_engine::Job {
+  int64_t id = 0;
+  std::vector<float> xyz, bond_r0, nat_r0;
+  std::vector<int32_t> chain, type, nat_i, nat_j;
+  sop_system_desc desc{};
+  int64_t nsteps = 0;
+  int64_t bytes = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  int32_t slot = -1;
+  int32_t status = SOP_OK;
+  int32_t done = 0;
+  double energies[SOP_E_COUNT] = {0};
+};
+
+struct sop_engine::Serve {
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool stop = false;
+  std::deque<Job *> queue;
+  std::unordered_map<int64_t, std::unique_ptr<Job>> jobs;
+  std::vector<Job *> running;
+  int64_t next_id = 0;
+  int64_t budget = 0, used = 0;
+  int32_t max_jobs = 0, block = 0, n_nl = 0, n_pl = 0;
+  bool schedule_set = false;
+  std::string fatal;
+};
+
+namespace {
+
+// Retire finished jobs: one energy launch for all of them, then coordinates
+// and energies back to the host, then free their device memory.
+sop_status retire(sop_engine *e, sop_engine::Serve *sv) {
+  std::vector<SysDev> sd((size_t)e->cap.slots);
+  sop_status st = hip_check(e, hipMemcpy(sd.data(), e->p.sys, sizeof(SysDev) * sd.size(), hipMemcpyDeviceToHost),
+                            "retire");
+  if (st) return st;
+  std::vector<sop_engine::Job *> keep, fin;
+  std::vector<int32_t> fin_slots;
+  for (sop_engine::Job *j : sv->running) {
+    const SysDev &S = sd[j->slot];
+    if (S.step >= S.step_end || S.err) {
+      fin.push_back(j);
+      fin_slots.push_back(j->slot);
+    } else {
+      keep.push_back(j);
+    }
+  }
+  if (fin.empty()) return SOP_OK;
+  st = engine_set_batch(e, fin_slots.data(), (int32_t)fin_slots.size());
+  if (st) return st;
+  st = hip_check(e, launch_energy(e->p, grid_tiles(e), grid_sys(e), e->stream), "energy");
+  if (st) return st;
+  std::vector<double> out(fin.size() * SYSOUT_N);
+  std::vector<std::vector<float4>> xs(fin.size());
+  for (size_t q = 0; q < fin.size(); ++q) {
+    const SysDev &S = sd[fin[q]->slot];
+    st = hip_check(e, hipMemcpyAsync(&out[q * SYSOUT_N], e->p.sysout + (int64_t)fin[q]->slot * SYSOUT_N,
+                                     8 * SYSOUT_N, hipMemcpyDeviceToHost, e->stream), "energy copy");
+    if (st) return st;
+    xs[q].resize((size_t)S.n);
+    st = hip_check(e, hipMemcpyAsync(xs[q].data(), (S.parity & 1 ? e->p.pos1 : e->p.pos0) + S.bead_off,
+                                     16 * (size_t)S.n, hipMemcpyDeviceToHost, e->stream), "coords copy");
+    if (st) return st;
+  }
+  st = hip_check(e, hipStreamSynchronize(e->stream), "retire sync");
+  if (st) return st;
+  const double now = sop_clock_seconds();
+  for (size_t q = 0; q < fin.size(); ++q) {
+    sop_engine::Job *j = fin[q];
+    const int32_t status = sd[j->slot].err ? engine_check_errors(e, &j->slot, 1) : SOP_OK;
+    engine_remove_system(e, j->slot);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t i = 0; i < xs[q].size(); ++i) {
+      j->xyz[3 * i] = xs[q][i].x;
+      j->xyz[3 * i + 1] = xs[q][i].y;
+      j->xyz[3 * i + 2] = xs[q][i].z;
+    }
+    for (int k = 0; k < SOP_E_COUNT; ++k) j->energies[k] = out[q * SYSOUT_N + k];
+    j->status = status;
+    j->t_done = now;
+    j->done = 1;
+    j->slot = -1;
+    sv->used -= j->bytes;
+  }
+  sv->running.swap(keep);
+  return SOP_OK;
 }
+
+void serve_loop(sop_engine *e) {
+  sop_engine::Serve *sv = e->serve;
+  (void)hipSetDevice(e->device);
+  for (;;) {
+    // 1. admission
+    std::vector<sop_engine::Job *> admit;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      if (sv->running.empty() && sv->queue.empty() && !sv->stop)
+        sv->cv.wait_for(lk, std::chrono::milliseconds(5));
+      if (sv->stop) break;
+      while (!sv->queue.empty()) {
+        sop_engine::Job *j = sv->queue.front();
+        if ((int)(sv->running.size() + admit.size()) >= sv->max_jobs) break;
+        if (sv->used + j->bytes > sv->budget && !(sv->running.empty() && admit.empty())) break;
+        sv->queue.pop_front();
+        sv->used += j->bytes;
+        admit.push_back(j);
+      }
+    }
+    for (sop_engine::Job *j : admit) {
+      int32_t slot = -1;
+      sop_status st = engine_add_system(e, &j->desc, &slot);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      j->t_admit = sop_clock_seconds();
+      if (st != SOP_OK) {
+        j->status = st;
+        j->done = 1;
+        j->t_done = j->t_admit;
+        sv->used -= j->bytes;
+        continue;
+      }
+      j->slot = slot;
+      // the job runs from its step0 to step0 + nsteps
+      const int64_t end = j->desc.step0 + j->nsteps;
+      if (hipMemcpy((char *)(e->p.sys + slot) + offsetof(SysDev, step_end), &end, 8, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+        sv->fatal = "step_end upload failed";
+        return;
+      }
+      sv->running.push_back(j);
+    }
+    if (sv->running.empty()) continue;
+    // 2. one block of steps over the running batch
+    std::vector<int32_t> slots;
+    for (sop_engine::Job *j : sv->running) slots.push_back(j->slot);
+    sop_status st = engine_set_batch(e, slots.data(), (int32_t)slots.size());
+    if (st == SOP_OK) st = engine_run_block_graph(e, sv->block);
+    // 3. retire finished jobs
+    if (st == SOP_OK) st = retire(e, sv);
+    if (st != SOP_OK) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->fatal = e->err;
+      for (sop_engine::Job *j : sv->running) {
+        j->status = st;
+        j->done = 1;
+        j->t_done = sop_clock_seconds();
+      }
+      sv->running.clear();
+      return;
+    }
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+sop_status sop_serve_start(sop_engine *e, const sop_serve_opts *o) {
+  if (!e) return SOP_ERR_INVALID;
+  if (e->serve) return set_error(e, SOP_ERR_STATE, "already serving");
+  auto *sv = new sop_engine::Serve();
+  sv->budget = (o && o->hbm_budget_bytes > 0) ? o->hbm_budget_bytes : (int64_t)8 << 30;
+  sv->max_jobs = (o && o->max_jobs > 0) ? o->max_jobs : 4096;
+  sv->block = (o && o->block_steps > 0) ? o->block_steps : 0;
+  e->serve = sv;
+  e->fixed_grids = true;
+  sv->th = std::thread(serve_loop, e);
+  return SOP_OK;
+}
+
+sop_status sop_serve_submit(sop_engine *e, const sop_system_desc *d, int64_t nsteps, int64_t *job) {
+  if (!e || !d || !job || nsteps < 0 || d->n_beads < 1 || !d->xyz) return SOP_ERR_INVALID;
+  sop_engine::Serve *sv = e->serve;
+  if (!sv) return SOP_ERR_STATE;
+  auto j = std::make_unique<sop_engine::Job>();
+  const int n = d->n_beads, m = d->n_native;
+  j->xyz.assign(d->xyz, d->xyz + 3 * (size_t)n);
+  if (d->chain) j->chain.assign(d->chain, d->chain + n);
+  if (d->type) j->type.assign(d->type, d->type + n);
+  if (d->bond_r0) j->bond_r0.assign(d->bond_r0, d->bond_r0 + n);
+  if (m > 0) {
+    j->nat_i.assign(d->nat_i, d->nat_i + m);
+    j->nat_j.assign(d->nat_j, d->nat_j + m);
+    j->nat_r0.assign(d->nat_r0, d->nat_r0 + m);
+  }
+  j->desc = *d;
+  j->desc.xyz = j->xyz.data();
+  j->desc.chain = d->chain ? j->chain.data() : nullptr;
+  j->desc.type = d->type ? j->type.data() : nullptr;
+  j->desc.bond_r0 = d->bond_r0 ? j->bond_r0.data() : nullptr;
+  j->desc.nat_i = m > 0 ? j->nat_i.data() : nullptr;
+  j->desc.nat_j = m > 0 ? j->nat_j.data() : nullptr;
+  j->desc.nat_r0 = m > 0 ? j->nat_r0.data() : nullptr;
+  j->nsteps = nsteps;
+  j->bytes = job_bytes_estimate(e, d);
+  j->t_submit = sop_clock_seconds();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (!sv->fatal.empty()) return SOP_ERR_STATE;
+  // one schedule per served batch: graph blocks hold the list rebuilds at
+  // fixed offsets, so every job shares n_nl / n_pl and starts on a block boundary
+  const sop_params &P = d->params;
+  if (!sv->schedule_set) {
+    if (P.n_pl < 1 || P.n_nl < 1 || P.n_nl % P.n_pl != 0)
+      return SOP_ERR_INVALID;
+    sv->n_nl = P.n_nl;
+    sv->n_pl = P.n_pl;
+    if (sv->block <= 0) sv->block = P.n_nl;
+    if (sv->block % P.n_pl != 0) return SOP_ERR_INVALID;
+    sv->schedule_set = true;
+  }
+  if (P.n_nl != sv->n_nl || P.n_pl != sv->n_pl)
+    return SOP_ERR_INVALID;
+  if (d->step0 % sv->block != 0) return SOP_ERR_INVALID;
+  j->id = sv->next_id++;
+  *job = j->id;
+  sv->queue.push_back(j.get());
+  sv->jobs[j->id] = std::move(j);
+  sv->cv.notify_one();
+  return SOP_OK;
+}
+
+sop_status sop_serve_poll(sop_engine *e, int64_t job, sop_job_result *r, float *xyz) {
+  if (!e || !r) return SOP_ERR_INVALID;
+  sop_engine::Serve *sv = e->serve;
+  if (!sv) return SOP_ERR_STATE;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->jobs.find(job);
+  if (it == sv->jobs.end()) return SOP_ERR_INVALID;
+  sop_engine::Job *j = it->second.get();
+  std::memset(r, 0, sizeof(*r));
+  r->done = j->done;
+  r->status = j->status;
+  r->n_beads = j->desc.n_beads;
+  r->steps = j->nsteps;
+  r->t_submit = j->t_submit;
+  r->t_admit = j->t_admit;
+  r->t_done = j->t_done;
+  if (!j->done) return SOP_ERR_BUSY;
+  for (int k = 0; k < SOP_E_COUNT; ++k) r->energies[k] = j->energies[k];
+  if (xyz) std::memcpy(xyz, j->xyz.data(), sizeof(float) * j->xyz.size());
+  return SOP_OK;
+}
+
+int64_t sop_serve_job_bytes(const sop_engine *e, const sop_system_desc *d) {
+  if (!e || !d || d->n_beads < 1) return -1;
+  return job_bytes_estimate(e, d);
+}
+
+sop_status sop_serve_stop(sop_engine *e) {
+  if (!e) return SOP_ERR_INVALID;
+  sop_engine::Serve *sv = e->serve;
+  if (!sv) return SOP_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv.notify_all();
+  if (sv->th.joinable()) sv->th.join();
+  // release whatever is still resident
+  for (sop_engine::Job *j : sv->running)
+    if (j->slot >= 0) engine_remove_system(e, j->slot);
+  const bool ok = sv->fatal.empty();
+  delete sv;
+  e->serve = nullptr;
+  e->fixed_grids = false;
+  return ok ? SOP_OK : SOP_ERR_STATE;
+}
+
+} // extern "C"

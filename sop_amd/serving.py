@@ -1,0 +1,122 @@
+"""Continuous-batching server (libsop serve loop) and the served benchmark leg.
+
+A Server owns an Engine in serving mode: jobs (System + step count) are
+submitted as they arrive; the engine's serve thread admits them by HBM budget,
+runs them in the resident batch in hipGraph blocks, and retires each as it
+finishes.  Latency is arrival (submit) to result available on the host.
+"""
+import ctypes as C
+import threading
+import time
+
+import numpy as np
+
+from . import Engine, System, SopError, _abi, _check, _ptr, lib, params_config
+from . import workloads as W
+
+
+class Server:
+    def __init__(self, engine: Engine, hbm_budget_bytes=0, max_jobs=0, block_steps=0):
+        self.eng = engine
+        o = _abi.ServeOpts()
+        o.hbm_budget_bytes, o.max_jobs, o.block_steps = hbm_budget_bytes, max_jobs, block_steps
+        engine._c(lib().sop_serve_start(engine._h, C.byref(o)))
+        self._n = {}
+
+    def job_bytes(self, system: System) -> int:
+        d = system.desc()
+        return lib().sop_serve_job_bytes(self.eng._h, C.byref(d))
+
+    def submit(self, system: System, nsteps: int) -> int:
+        d = system.desc()
+        j = C.c_int64()
+        _check(lib().sop_serve_submit(self.eng._h, C.byref(d), nsteps, C.byref(j)))
+        self._n[j.value] = system.n
+        return j.value
+
+    def poll(self, job, with_coords=False):
+        """Returns (done, result dict, coords or None)."""
+        r = _abi.JobResult()
+        x = np.zeros((self._n[job], 3), np.float32) if with_coords else None
+        st = lib().sop_serve_poll(self.eng._h, job, C.byref(r), _ptr(x, C.c_float) if x is not None else None)
+        if st == _abi.SOP_ERR_BUSY:
+            return False, None, None
+        _check(st)
+        res = {"status": r.status, "n_beads": r.n_beads, "steps": r.steps, "t_submit": r.t_submit,
+               "t_admit": r.t_admit, "t_done": r.t_done, "energies": np.array(r.energies[:])}
+        return True, res, x
+
+    def wait(self, job, timeout=600.0, with_coords=True):
+        t0 = time.time()
+        while True:
+            done, res, x = self.poll(job, with_coords)
+            if done:
+                return res, x
+            if time.time() - t0 > timeout:
+                raise TimeoutError(f"job {job} not done after {timeout}s")
+            time.sleep(0.001)
+
+    def stop(self):
+        self.eng._c(lib().sop_serve_stop(self.eng._h))
+
+
+def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates=256, seed=4,
+                 budget_gib=32.0, drain_timeout=120.0):
+    """Served workload: Poisson arrivals at `rate` jobs/s for `seconds`; job
+    sizes log-normal (median 1500, sigma 0.9) clipped to [100, 20000]; steps
+    log-uniform in [steps_lo, steps_hi]; T uniform in [0.5, 0.9]; fresh Philox
+    key per job.  Chains come from `templates` pre-generated compact walks of
+    the drawn sizes (job k uses template k % templates).  Returns latency
+    percentiles and aggregate bead-steps/s of the completed jobs."""
+    jobs = W.c4_jobs(rate=rate, duration=seconds, seed=seed, steps_lo=steps_lo, steps_hi=steps_hi)
+    t0 = time.perf_counter()
+    base = [W.c4_system(jobs[k], seed) for k in range(min(templates, len(jobs)))]
+    log(f"served: {len(jobs)} arrivals over {seconds}s, {len(base)} chain templates in "
+        f"{time.perf_counter() - t0:.1f}s")
+    eng = Engine(0)
+    srv = Server(eng, hbm_budget_bytes=int(budget_gib * (1 << 30)))
+    ids = []
+    start = time.perf_counter()
+    clock0 = lib().sop_clock_seconds()
+    for (t, nb, nst, T, k) in jobs:
+        now = time.perf_counter() - start
+        if t > now:
+            time.sleep(t - now)
+        s = base[k % len(base)]
+        s.params.T = T
+        s.params.seed = W.philox_key(seed, k)
+        ids.append((srv.submit(s, nst), s.n, nst))
+    log(f"served: all {len(ids)} jobs submitted at {time.perf_counter() - start:.1f}s")
+    lat, done_beadsteps, t_last = [], 0, clock0
+    pending = list(ids)
+    deadline = time.perf_counter() + drain_timeout
+    while pending and time.perf_counter() < deadline:
+        keep = []
+        for (j, n, nst) in pending:
+            done, res, _ = srv.poll(j)
+            if done:
+                if res["status"] != 0:
+                    raise SopError(res["status"], f"served job {j} failed")
+                lat.append(res["t_done"] - res["t_submit"])
+                done_beadsteps += n * nst
+                t_last = max(t_last, res["t_done"])
+            else:
+                keep.append((j, n, nst))
+        pending = keep
+        if pending:
+            time.sleep(0.01)
+    srv.stop()
+    eng.close()
+    wall = t_last - clock0
+    lat = np.array(lat)
+    offered = sum(n * nst for (_, n, nst) in ids) / seconds
+    out = {"jobs": len(ids), "completed": int(len(lat)), "arrival_rate": rate, "arrival_seconds": seconds,
+           "steps_range": [steps_lo, steps_hi], "templates": len(base),
+           "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+           "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+           "bead_steps_per_s": done_beadsteps / wall if wall > 0 else None,
+           "offered_bead_steps_per_s": offered, "wall_s": wall}
+    log(f"served: {out['completed']}/{out['jobs']} done, p50 {out['p50_latency_s']:.3f}s "
+        f"p99 {out['p99_latency_s']:.3f}s, {out['bead_steps_per_s']:.4g} bead-steps/s "
+        f"(offered {offered:.4g})")
+    return out

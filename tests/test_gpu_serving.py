@@ -1,0 +1,98 @@
+"""Continuous-batching serve loop on device 0.
+
+- every job's final coordinates and energies are identical (bitwise) to the
+  same job run alone on the plain engine API;
+- jobs submitted while others run are admitted and overlap with them;
+- the job state resident at any time stays within the HBM budget;
+- jobs that break the shared schedule are refused at submission.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import sop_amd as S
+from sop_amd import serving
+from sop_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def job_systems():
+    out = []
+    for k, (n, steps) in enumerate([(150, 120), (900, 300), (2500, 200), (64, 50), (1200, 400), (3000, 100),
+                                    (400, 250), (700, 0), (1800, 150), (260, 500)]):
+        s = S.make_chain_system(500 + k, n)
+        s.params.seed = W.philox_key(9, k)
+        s.params.T = 0.5 + 0.04 * k
+        out.append((s, steps))
+    return out
+
+
+def alone(s, steps):
+    with S.Engine(0) as e:
+        i = e.add(s)
+        if steps:
+            e.run([i], steps)
+        return e.coords(i), e.energy_eval(i)
+
+
+def test_served_results_equal_alone_and_overlap():
+    jobs = job_systems()
+    with S.Engine(0) as eng:
+        srv = serving.Server(eng)
+        ids = []
+        for s, steps in jobs[:5]:
+            ids.append(srv.submit(s, steps))
+        time.sleep(0.05)  # the first jobs are running when the rest arrive
+        for s, steps in jobs[5:]:
+            ids.append(srv.submit(s, steps))
+        res = [srv.wait(j) for j in ids]
+        srv.stop()
+    for (s, steps), (r, x) in zip(jobs, res):
+        assert r["status"] == 0
+        xr, er = alone(s, steps)
+        assert np.array_equal(x, xr), (s.n, steps)
+        assert np.array_equal(r["energies"], er[:6])
+    # continuous batching: some job was admitted while another was resident
+    iv = sorted((r["t_admit"], r["t_done"]) for r, _ in res)
+    assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
+
+
+def test_admission_respects_hbm_budget():
+    jobs = job_systems()
+    with S.Engine(0) as eng:
+        probe = serving.Server(eng)
+        sizes = [probe.job_bytes(s) for s, _ in jobs]
+        probe.stop()
+        budget = int(sorted(sizes)[-1] + sorted(sizes)[-2])  # two largest jobs
+        srv = serving.Server(eng, hbm_budget_bytes=budget)
+        ids = [srv.submit(s, steps) for s, steps in jobs]
+        res = [srv.wait(j, with_coords=False)[0] for j in ids]
+        srv.stop()
+    events = []
+    for r, b in zip(res, sizes):
+        assert r["status"] == 0
+        events += [(r["t_admit"], b), (r["t_done"], -b)]
+    events.sort(key=lambda ev: (ev[0], ev[1]))
+    resident = peak = 0
+    for _, b in events:
+        resident += b
+        peak = max(peak, resident)
+    assert peak <= budget
+
+
+def test_schedule_mismatch_is_refused():
+    with S.Engine(0) as eng:
+        srv = serving.Server(eng)
+        a = S.make_chain_system(1, 100)
+        srv.submit(a, 10)
+        b = S.make_chain_system(2, 100)
+        b.params.n_pl = 5
+        with pytest.raises(S.SopError):
+            srv.submit(b, 10)
+        c = S.make_chain_system(3, 100)
+        c.step0 = 7
+        with pytest.raises(S.SopError):
+            srv.submit(c, 10)
+        srv.stop()
