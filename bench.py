@@ -83,6 +83,36 @@ def cpu_baseline(systems, steps=300, warm=20, max_threads=16):
             "seconds": dt}
 
 
+# HBM bytes per launch of the step kernel from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over this workload (profiles/k_step_batched_512_replicas.md),
+# each scaled by the counter's reading for a device copy of known size
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Keyed by (kernel, replicas, beads);
+# any other run reports null.
+TRAFFIC = {
+    ("k_step<8,0>", 512, 737499): 387.19e6 + 11.80e6,
+}
+
+
+def measured_traffic(kernel, replicas, beads):
+    return TRAFFIC.get((kernel, replicas, beads))
+
+
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size (the runtime's copy kernel),
+    so that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copied {nbytes} bytes device-to-device 3 times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +126,12 @@ def main():
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--lds-beads", type=int, default=0, help="LDS-resident step for systems up to this size "
                     "(0 = engine default, < 0 = off)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (counter calibration for profiles)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(replicas would shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
@@ -133,12 +168,15 @@ def main():
     achieved = step_bytes / (avg_ms * 1e-3)
     log(f"step kernel {avg_ms:.4f} ms/launch, {achieved / 1e9:.1f} GB/s algorithmic")
 
-    t0 = time.perf_counter()
-    eng.run_plain(ids, args.plain_steps)
-    pdt = time.perf_counter() - t0
-    plain_value = beads * args.plain_steps / pdt
-    log(f"plain path {args.plain_steps} steps: {pdt:.3f}s -> {plain_value:.4g} bead-steps/s")
+    plain_value = None
+    if args.plain_steps > 0:
+        t0 = time.perf_counter()
+        eng.run_plain(ids, args.plain_steps)
+        pdt = time.perf_counter() - t0
+        plain_value = beads * args.plain_steps / pdt
+        log(f"plain path {args.plain_steps} steps: {pdt:.3f}s -> {plain_value:.4g} bead-steps/s")
     eng.close()
+    traffic = measured_traffic(f"k_step<{lanes},0>", len(systems), beads)
 
     served = None
     if args.serve_seconds > 0:
@@ -173,14 +211,14 @@ def main():
             "parallelism": "replica batch on 1 GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": f"k_step<{lanes},0>", "kernel_ms": avg_ms,
                      "algorithmic_bytes_per_launch": step_bytes},
         "schedule_roofline": {"bytes_per_bead_step": bytes_per_bead_step,
                               "frac": value * bytes_per_bead_step / HBM_PEAK},
         "plain": {"value": plain_value, "steps": args.plain_steps,
-                  "path": "one system per launch, no graphs, same kernels"},
-        "batched_vs_plain": value / plain_value,
+                  "path": "one system per launch, no graphs, same kernels"} if plain_value else None,
+        "batched_vs_plain": value / plain_value if plain_value else None,
         "served": served,
         "cpu_baseline": cpu,
     }
