@@ -15,7 +15,8 @@
  *   list entry points          sop.h:32-35 (update_neighbor_list,
  *                              update_pair_list, update_cell_list,
  *                              update_hybrid_list)
- *   integrator                 sop.h:28 (overdamped_iteration)
+ *   integrators                sop.h:27-28 (underdamped_iteration,
+ *                              overdamped_iteration)
  *   device state               GPUvars.h:150-153 (alloc_GPU_arrays,
  *                              alloc_cudpp, setup_rng)
  * The reference passes all data through globals (global.h:83-233); here every
@@ -59,6 +60,10 @@ typedef int32_t sop_status;
 #define SOP_LIST_PL_NATIVE 2 /* idx_pair_list_att     (global.h:170) */
 #define SOP_LIST_PL_REP 3    /* idx_pair_list_rep     (global.h:172) */
 
+/* dynamics (sim_type, param.cpp:239-254) */
+#define SOP_DYN_OVERDAMPED 0  /* sim_type 2: x += (h/zeta)(F + noise) (sop.cu:487-520) */
+#define SOP_DYN_UNDERDAMPED 1 /* sim_type 1: Langevin leapfrog with a1..a4 (sop.cu:421-484) */
+
 typedef struct sop_engine sop_engine;
 
 /* Force field and run parameters of one system.  Bead types are 1 or 2 as in
@@ -87,6 +92,8 @@ typedef struct sop_params {
   float h;        /* time step, tau_L (param.cpp:293) */
   float boxl;     /* periodic box edge, A; <= 0 disables the minimum image */
   uint64_t seed;  /* Philox4x32-10 key of this system's random force stream */
+  int32_t dynamics; /* SOP_DYN_OVERDAMPED or SOP_DYN_UNDERDAMPED */
+  int32_t reserved; /* 0 */
 } sop_params;
 
 /* One system: linear chains of beads.  Beads of a chain are contiguous and
@@ -107,6 +114,8 @@ typedef struct sop_system_desc {
   const float *nat_r0;    /* native distance (lj_nat_pdb_dist, io.cpp:208) */
   sop_params params;
   int64_t step0;          /* initial value of the step counter */
+  const float *vel;       /* underdamped: 3*n velocities entering the first step
+                             (vel, global.h:176); NULL = 0 */
 } sop_system_desc;
 
 typedef struct sop_engine_opts {
@@ -131,6 +140,10 @@ sop_status sop_params_config(sop_params *p);
 /* Derived FP32 coefficients exactly as the engine uses them:
  * a5 = h/zeta (param.cpp:299), var = sqrt(2 T zeta / h) (energy.cu:158). */
 sop_status sop_params_derived(const sop_params *p, float *a5, float *var);
+/* Underdamped coefficients a[0..3] = a1..a4 with the reference's float/double
+ * evaluation order (param.cpp:104-108): a1 = h(1 - zeta h/2), a2 = h h/2,
+ * a3 = (1 - h zeta/2 + (h zeta)^2/4)/h, a4 = (h/2)(1 - h zeta/2). */
+sop_status sop_params_underdamped(const sop_params *p, float *a4);
 
 /* --- engine -------------------------------------------------------------- */
 void sop_engine_opts_default(sop_engine_opts *o);
@@ -179,6 +192,14 @@ sop_status sop_get_coords(sop_engine *e, int32_t sys, float *xyz, int64_t *step)
 sop_status sop_set_coords(sop_engine *e, int32_t sys, const float *xyz);
 /* Wrapped coordinates pos = unc - boxl*rint(unc/boxl) (sop.cu:510-512). */
 sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz);
+/* Velocities as the reference holds them after its last iteration (vel,
+ * update_velocities_kernel, sop.cu:455-467): underdamped a3*incr + a4*F with F
+ * the force (with noise) at the current coordinates and step, or the frozen
+ * velocities when T < 1e-5; overdamped: zeros (sop.cu:296-305). */
+sop_status sop_get_velocities(sop_engine *e, int32_t sys, float *v3n);
+/* The position increment of the last step (dev_incr, used by kinT in
+ * calculate_observables, sop.cu:596-606).  Valid after at least one step. */
+sop_status sop_get_increments(sop_engine *e, int32_t sys, float *d3n);
 
 /* --- measurement --------------------------------------------------------- */
 typedef struct sop_list_stats {

@@ -47,6 +47,9 @@ def lib():
         _lib.orc_run.argtypes = [P(OrcSys), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64, C.c_float,
                                  C.c_float, C.c_int32, C.c_int32, P(C.c_double)]
         _lib.orc_port_create.argtypes = [P(OrcSys)]
+        _lib.orc_run_ud.argtypes = [P(OrcSys), P(C.c_float), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64,
+                                    P(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
+                                    P(C.c_double), P(C.c_double)]
         _lib.orc_port_create.restype = C.c_void_p
         _lib.orc_port_destroy.argtypes = [C.c_void_p]
         _lib.orc_port_run.argtypes = [C.c_void_p, P(C.c_float), C.c_int64, C.c_int64, C.c_uint64, C.c_float,
@@ -190,6 +193,21 @@ class Oracle:
                            _p(out, C.c_double))
         assert st == 0, st
         return out.reshape(-1, 3)
+
+
+    def run_ud(self, x0, step0, nsteps, seed, a, var, vupd, n_nl, n_pl, v0=None):
+        """FP64 underdamped trajectory; returns (coordinates, velocities after
+        the last step), each (n,3) float64."""
+        x = np.ascontiguousarray(x0, np.float32).reshape(-1)
+        v = None if v0 is None else np.ascontiguousarray(v0, np.float32).reshape(-1)
+        aa = np.ascontiguousarray(a, np.float32)
+        out = np.zeros(3 * self.n, np.float64)
+        vout = np.zeros(3 * self.n, np.float64)
+        st = lib().orc_run_ud(C.byref(self.s), _p(x, C.c_float), _p(v, C.c_float) if v is not None else None,
+                              step0, nsteps, seed, _p(aa, C.c_float), var, int(vupd), n_nl, n_pl,
+                              _p(out, C.c_double), _p(vout, C.c_double))
+        assert st == 0, st
+        return out.reshape(-1, 3), vout.reshape(-1, 3)
 
 
 class Port:

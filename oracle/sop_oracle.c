@@ -450,6 +450,85 @@ ORC_EXPORT int orc_run(const orc_sys *s, const float *x0, int64_t step0, int64_t
   return st;
 }
 
+/* Underdamped Langevin dynamics (sop.cu:421-484, coefficients
+ * param.cpp:104-108) in FP64.  The reference computes incr = a1 v + a2 F,
+ * moves, evaluates F at the new coordinates, then v = a3 incr + a4 F (no
+ * velocity update when T < 1e-5, sop.cu:475).  Restated per step s with the
+ * lists rebuilt before the force evaluation as in orc_run:
+ *   F_s = F(x_s) + var * xi(seed, bead, s)
+ *   v_s = a3 u + a4 F_s    if u holds the previous increment, else v_s = u
+ *   incr = a1 v_s + a2 F_s;  x_{s+1} = x_s + incr
+ *   u = incr (velocities updated) or v_s (frozen).
+ * v0: velocities entering the first step (NULL = 0).  v_out: the reference's
+ * velocities after the last step (a3 u + a4 F_end), with F_end evaluated at
+ * the final coordinates, step step0 + nsteps, on the lists of the last step. */
+ORC_EXPORT int orc_run_ud(const orc_sys *s, const float *x0, const float *v0, int64_t step0, int64_t nsteps,
+                          uint64_t seed, const float *a, float var, int32_t vupd, int32_t n_nl, int32_t n_pl,
+                          double *x_out, double *v_out) {
+  int n = s->n;
+  double *x = x_out;
+  float *xf = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+  double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *u = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  orc_lists L;
+  int st = lists_alloc(s, &L);
+  if (!xf || !f || !u || st) { free(xf); free(f); free(u); lists_free(&L); return -2; }
+  for (int i = 0; i < 3 * n; ++i) {
+    x[i] = (double)x0[i];
+    u[i] = v0 ? (double)v0[i] : 0.0;
+  }
+  int have_incr = 0;
+  const double A1 = a[0], A2 = a[1], A3 = a[2], A4 = a[3], VAR = var;
+  double e[6];
+  for (int64_t it = 0; it < nsteps; ++it) {
+    int64_t step = step0 + it;
+    for (int i = 0; i < 3 * n; ++i) xf[i] = (float)x[i];
+    if (it == 0 || step % n_nl == 0) {
+      st = orc_nl(s, xf, L.nla_k, s->n_nat + 1, &L.nla, L.nlr_i, L.nlr_j, L.cap_rep, &L.nlr);
+      if (st) break;
+    }
+    if (it == 0 || step % n_pl == 0)
+      orc_pl(s, xf, L.nla_k, L.nla, L.nlr_i, L.nlr_j, L.nlr, L.pla_k, &L.pla, L.plr_i, L.plr_j, &L.plr);
+    st = orc_forces(s, x, L.pla_k, L.pla, L.plr_i, L.plr_j, L.plr, f, e);
+    if (st) break;
+    for (int i = 0; i < n; ++i) {
+      double xi[3];
+      orc_normals(seed, (uint32_t)i, (uint64_t)step, xi);
+      for (int d = 0; d < 3; ++d) {
+        const int k = 3 * i + d;
+        const double F = f[k] + VAR * xi[d];
+        const double v = have_incr ? A3 * u[k] + A4 * F : u[k];
+        const double incr = A1 * v + A2 * F;
+        x[k] += incr;
+        u[k] = vupd ? incr : v;
+      }
+    }
+    have_incr = vupd;
+  }
+  if (!st && v_out) {
+    if (have_incr) {
+      if (nsteps == 0) {
+        for (int i = 0; i < 3 * n; ++i) xf[i] = (float)x[i];
+        orc_nl(s, xf, L.nla_k, s->n_nat + 1, &L.nla, L.nlr_i, L.nlr_j, L.cap_rep, &L.nlr);
+        orc_pl(s, xf, L.nla_k, L.nla, L.nlr_i, L.nlr_j, L.nlr, L.pla_k, &L.pla, L.plr_i, L.plr_j, &L.plr);
+      }
+      st = orc_forces(s, x, L.pla_k, L.pla, L.plr_i, L.plr_j, L.plr, f, e);
+      for (int i = 0; i < n && !st; ++i) {
+        double xi[3];
+        orc_normals(seed, (uint32_t)i, (uint64_t)(step0 + nsteps), xi);
+        for (int d = 0; d < 3; ++d) {
+          const int k = 3 * i + d;
+          v_out[k] = A3 * u[k] + A4 * (f[k] + VAR * xi[d]);
+        }
+      }
+    } else {
+      for (int i = 0; i < 3 * n; ++i) v_out[i] = u[i];
+    }
+  }
+  free(xf); free(f); free(u); lists_free(&L);
+  return st;
+}
+
 /* ------------------------------------------------------------------ */
 /* FP32 port of the reference's host method (CPU baseline).             */
 /* Explicit O(N^2) master non-native list (as loaded from the vdw file, */

@@ -76,6 +76,13 @@ def params_derived(p: Params):
     return np.float32(a5.value), np.float32(var.value)
 
 
+def params_underdamped(p: Params):
+    """a1..a4 of the underdamped integrator as float32 (param.cpp:104-108)."""
+    a = (C.c_float * 4)()
+    _check(lib().sop_params_underdamped(C.byref(p), a))
+    return np.array(a[:], np.float32)
+
+
 def copy_params(p: Params) -> Params:
     q = Params()
     C.memmove(C.byref(q), C.byref(p), C.sizeof(Params))
@@ -94,6 +101,7 @@ class System:
     nat_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     nat_r0: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
     step0: int = 0
+    vel: Optional[np.ndarray] = None     # (n,3) float32 underdamped initial velocities
 
     def __post_init__(self):
         self.xyz = np.ascontiguousarray(self.xyz, dtype=np.float32).reshape(-1, 3)
@@ -106,6 +114,8 @@ class System:
         self.nat_i = np.ascontiguousarray(self.nat_i, np.int32)
         self.nat_j = np.ascontiguousarray(self.nat_j, np.int32)
         self.nat_r0 = np.ascontiguousarray(self.nat_r0, np.float32)
+        if self.vel is not None:
+            self.vel = np.ascontiguousarray(self.vel, np.float32).reshape(-1, 3)
 
     @property
     def n(self):
@@ -124,6 +134,7 @@ class System:
         d.nat_r0 = _ptr(self.nat_r0, C.c_float)
         d.params = self.params
         d.step0 = self.step0
+        d.vel = _ptr(self.vel, C.c_float) if self.vel is not None else None
         return d
 
     @staticmethod
@@ -287,6 +298,16 @@ class Engine:
         x = np.zeros((self._n[sys], 3), np.float32)
         self._c(lib().sop_get_wrapped_coords(self._h, sys, _ptr(x, C.c_float)))
         return x
+
+    def velocities(self, sys) -> np.ndarray:
+        v = np.zeros((self._n[sys], 3), np.float32)
+        self._c(lib().sop_get_velocities(self._h, sys, _ptr(v, C.c_float)))
+        return v
+
+    def increments(self, sys) -> np.ndarray:
+        d = np.zeros((self._n[sys], 3), np.float32)
+        self._c(lib().sop_get_increments(self._h, sys, _ptr(d, C.c_float)))
+        return d
 
     def set_coords(self, sys, xyz):
         x = np.ascontiguousarray(xyz, np.float32)

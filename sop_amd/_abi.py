@@ -23,6 +23,7 @@ SOP_ERR_PBC = -9
 E_BOND, E_ANGLE, E_NATIVE, E_REP, E_TOTAL, E_VIRIAL = range(6)
 E_COUNT = 6
 LIST_NL_NATIVE, LIST_NL_REP, LIST_PL_NATIVE, LIST_PL_REP = range(4)
+DYN_OVERDAMPED, DYN_UNDERDAMPED = 0, 1
 
 F3x3 = (C.c_float * 3) * 3
 
@@ -37,6 +38,7 @@ class Params(C.Structure):
         ("n_nl", C.c_int32), ("n_pl", C.c_int32),
         ("T", C.c_float), ("zeta", C.c_float), ("h", C.c_float), ("boxl", C.c_float),
         ("seed", C.c_uint64),
+        ("dynamics", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -53,6 +55,7 @@ class SystemDesc(C.Structure):
         ("nat_r0", C.POINTER(C.c_float)),
         ("params", Params),
         ("step0", C.c_int64),
+        ("vel", C.POINTER(C.c_float)),
     ]
 
 
@@ -97,6 +100,7 @@ SIGNATURES = {
     "sop_params_reference": (i32, [P(Params)]),
     "sop_params_config": (i32, [P(Params)]),
     "sop_params_derived": (i32, [P(Params), P(f32), P(f32)]),
+    "sop_params_underdamped": (i32, [P(Params), P(f32)]),
     "sop_engine_opts_default": (None, [P(EngineOpts)]),
     "sop_engine_create": (i32, [i32, P(EngineOpts), P(vp)]),
     "sop_engine_destroy": (None, [vp]),
@@ -118,6 +122,8 @@ SIGNATURES = {
     "sop_get_coords": (i32, [vp, i32, P(f32), P(i64)]),
     "sop_set_coords": (i32, [vp, i32, P(f32)]),
     "sop_get_wrapped_coords": (i32, [vp, i32, P(f32)]),
+    "sop_get_velocities": (i32, [vp, i32, P(f32)]),
+    "sop_get_increments": (i32, [vp, i32, P(f32)]),
     "sop_list_stats_get": (i32, [vp, i32, P(ListStats)]),
     "sop_time_step_kernel": (i32, [vp, P(i32), i32, i32, P(f64)]),
     "sop_autotune": (i32, [vp, P(i32), i32, P(i32), P(i32)]),

@@ -105,8 +105,6 @@ int grid_items(const sop_engine *e) {
 
 // ----------------------------------------------------------------- pools
 
-static size_t bead_elem_bytes[] = {16, 16, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 16, 48};
-
 static void bind_arrays(sop_engine *e) {
   DevPtrs &p = e->p;
   e->bead_arrays = {
@@ -115,12 +113,11 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.nat_start, 4},  {(void **)&p.nat_cnt, 4},     {(void **)&p.nlatt_cnt, 4},
       {(void **)&p.platt_cnt, 4},  {(void **)&p.nlrep_cnt, 4},   {(void **)&p.plrep_cnt, 4},
       {(void **)&p.cell_of, 4},    {(void **)&p.cell_tmp, 4},    {(void **)&p.cell_sorted, 4},
-      {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}};
+      {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},   {(void **)&p.nat_r0, 4},   {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
                     {(void **)&p.cellitems, 4}};
-  (void)bead_elem_bytes;
 }
 
 static sop_status grow_array(sop_engine *e, void **ptr, size_t old_bytes, size_t new_bytes) {
@@ -256,7 +253,9 @@ int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
   const int64_t pl_cap = round_up((int32_t)std::max<int64_t>(1, std::min<int64_t>(n - 1, e->opts.pl_cap_max)), 16);
   const int64_t beads = (n + TILE - 1) / TILE * TILE;
   const int64_t cell_cap = std::min<int64_t>(e->opts.cell_cap_max, std::max<int64_t>(64, 2 * n)) + 1;
-  return beads * 152 + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + pl_cap) + 12 * cell_cap;
+  int64_t per_bead = 4;  // tile table share
+  for (const auto &a : e->bead_arrays) per_bead += (int64_t)a.second;
+  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + pl_cap) + 12 * cell_cap;
 }
 
 // Validate and upload one system; builds its initial neighbor and pair lists.
@@ -265,8 +264,11 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   const int32_t n = d->n_beads;
   const sop_params &P = d->params;
   if (P.n_nl < 1 || P.n_pl < 1) return set_error(e, SOP_ERR_INVALID, "n_nl and n_pl must be >= 1");
-  if (!(P.zeta > 0.0f) || !(P.h > 0.0f) || !(P.T >= 0.0f) || !(P.R0 > 0.0f))
-    return set_error(e, SOP_ERR_INVALID, "need zeta > 0, h > 0, T >= 0, R0 > 0");
+  if (P.dynamics != SOP_DYN_OVERDAMPED && P.dynamics != SOP_DYN_UNDERDAMPED)
+    return set_error(e, SOP_ERR_INVALID, "dynamics must be SOP_DYN_OVERDAMPED or SOP_DYN_UNDERDAMPED");
+  const bool under = P.dynamics == SOP_DYN_UNDERDAMPED;
+  if (!(under ? P.zeta >= 0.0f : P.zeta > 0.0f) || !(P.h > 0.0f) || !(P.T >= 0.0f) || !(P.R0 > 0.0f))
+    return set_error(e, SOP_ERR_INVALID, "need zeta > 0 (>= 0 underdamped), h > 0, T >= 0, R0 > 0");
   if (!(P.nl_factor > 0.0) && !(P.nl_abs > 0.0f)) return set_error(e, SOP_ERR_INVALID, "neighbor-list cutoff");
   if (!(P.pl_factor > 0.0) && !(P.pl_abs > 0.0f)) return set_error(e, SOP_ERR_INVALID, "pair-list cutoff");
   // chains and types
@@ -387,6 +389,13 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   HIPCK(hipMemcpy(p.nat_cnt + b0, ncnt.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
   for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt})
     HIPCK(hipMemcpy(arr + b0, zeros.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
+  {
+    // underdamped state: the velocities entering the first step (w = 0)
+    std::vector<float4> v((size_t)h.bead_len, make_float4(0, 0, 0, 0));
+    if (d->vel)
+      for (int i = 0; i < n; ++i) v[i] = make_float4(d->vel[3 * i], d->vel[3 * i + 1], d->vel[3 * i + 2], 0.0f);
+    HIPCK(hipMemcpy(p.vel + b0, v.data(), 16 * h.bead_len, hipMemcpyHostToDevice));
+  }
   HIPCK(hipMemcpy(p.nat_j + h.nat_off, natj.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(p.nat_r0 + h.nat_off, natr.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
   std::vector<int32_t> ts((size_t)(h.bead_len / TILE), slot);
@@ -411,6 +420,16 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   float a5, var;
   sop_params_derived(&P, &a5, &var);
   S.a5 = a5;
+  S.dyn = under ? 1 : 0;
+  S.vupd = P.T >= 1.0e-5f ? 1 : 0;  // sop.cu:475 (no velocity update for steepest descent)
+  {
+    float a[4];
+    sop_params_underdamped(&P, a);
+    S.a1 = a[0];
+    S.a2 = a[1];
+    S.a3 = a[2];
+    S.a4 = a[3];
+  }
   S.var = var;
   S.boxl = P.boxl;
   S.inv_boxl = P.boxl > 0.0f ? 1.0f / P.boxl : 0.0f;
@@ -1116,6 +1135,102 @@ sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz) {
       if (boxl > 0.0f) v[d] = v[d] - boxl * rintf(v[d] / boxl);
       xyz[3 * i + d] = v[d];
     }
+  }
+  return SOP_OK;
+}
+
+// Forces with noise of the current step (kstep 0) or of the last step taken
+// (kstep -1: the other coordinate buffer and step - 1) into fout, copied out.
+static sop_status noisy_forces(sop_engine *e, int32_t sys, int kstep, std::vector<float4> &f) {
+  sop_status st = engine_set_batch(e, &sys, 1);
+  if (st) return st;
+  HIPCK(launch_step(e->p, MODE_FORCE_NOISE, e->opts.lanes_per_bead, grid_items(e), kstep, e->cur_lds, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  st = engine_check_errors(e, &sys, 1);
+  if (st) return st;
+  const SysHost &h = e->sys[sys];
+  f.resize((size_t)h.n);
+  HIPCK(hipMemcpy(f.data(), e->p.fout + h.bead_off, 16 * (size_t)h.n, hipMemcpyDeviceToHost));
+  return SOP_OK;
+}
+
+static sop_status read_vel_state(sop_engine *e, int32_t sys, std::vector<float4> &v) {
+  HIPCK(hipStreamSynchronize(e->stream));
+  const SysHost &h = e->sys[sys];
+  v.resize((size_t)h.n);
+  HIPCK(hipMemcpy(v.data(), e->p.vel + h.bead_off, 16 * (size_t)h.n, hipMemcpyDeviceToHost));
+  return SOP_OK;
+}
+
+static bool incr_flag(const float4 &s) {
+  int32_t w;
+  std::memcpy(&w, &s.w, 4);
+  return w == 1;
+}
+
+sop_status sop_get_velocities(sop_engine *e, int32_t sys, float *v3n) {
+  if (!valid_sys(e, sys) || !v3n) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  const SysHost &h = e->sys[sys];
+  if (h.params.dynamics != SOP_DYN_UNDERDAMPED) {
+    std::fill(v3n, v3n + 3 * (size_t)h.n, 0.0f);
+    return SOP_OK;
+  }
+  std::vector<float4> v, f;
+  sop_status st = read_vel_state(e, sys, v);
+  if (st) return st;
+  bool any = false;
+  for (const auto &s : v) any |= incr_flag(s);
+  if (any && (st = noisy_forces(e, sys, 0, f))) return st;
+  float a[4];
+  sop_params_underdamped(&h.params, a);
+  for (int i = 0; i < h.n; ++i) {
+    const float4 s = v[i];
+    if (incr_flag(s)) {
+      v3n[3 * i] = a[2] * s.x + a[3] * f[i].x;
+      v3n[3 * i + 1] = a[2] * s.y + a[3] * f[i].y;
+      v3n[3 * i + 2] = a[2] * s.z + a[3] * f[i].z;
+    } else {
+      v3n[3 * i] = s.x;
+      v3n[3 * i + 1] = s.y;
+      v3n[3 * i + 2] = s.z;
+    }
+  }
+  return SOP_OK;
+}
+
+sop_status sop_get_increments(sop_engine *e, int32_t sys, float *d3n) {
+  if (!valid_sys(e, sys) || !d3n) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  const SysHost &h = e->sys[sys];
+  std::vector<float4> v, f;
+  sop_status st;
+  if (h.params.dynamics == SOP_DYN_UNDERDAMPED) {
+    if ((st = read_vel_state(e, sys, v))) return st;
+    bool all = true;
+    for (const auto &s : v) all &= incr_flag(s);
+    if (!all && (st = noisy_forces(e, sys, -1, f))) return st;
+    float a[4];
+    sop_params_underdamped(&h.params, a);
+    for (int i = 0; i < h.n; ++i) {
+      const float4 s = v[i];
+      if (incr_flag(s)) {
+        d3n[3 * i] = s.x;
+        d3n[3 * i + 1] = s.y;
+        d3n[3 * i + 2] = s.z;
+      } else {
+        d3n[3 * i] = a[0] * s.x + a[1] * f[i].x;
+        d3n[3 * i + 1] = a[0] * s.y + a[1] * f[i].y;
+        d3n[3 * i + 2] = a[0] * s.z + a[1] * f[i].z;
+      }
+    }
+    return SOP_OK;
+  }
+  if ((st = noisy_forces(e, sys, -1, f))) return st;
+  float a5, var;
+  sop_params_derived(&h.params, &a5, &var);
+  for (int i = 0; i < h.n; ++i) {
+    d3n[3 * i] = a5 * f[i].x;
+    d3n[3 * i + 1] = a5 * f[i].y;
+    d3n[3 * i + 2] = a5 * f[i].z;
   }
   return SOP_OK;
 }

@@ -79,6 +79,22 @@ sop_status sop_params_derived(const sop_params *p, float *a5, float *var) {
   return SOP_OK;
 }
 
+sop_status sop_params_underdamped(const sop_params *p, float *a) {
+  if (!p || !a) return SOP_ERR_INVALID;
+  // Products of two floats round to float; every expression that meets a
+  // double literal continues in double and is stored to float at the end.
+  const float h = p->h, zeta = p->zeta;
+  const float halfh = (float)((double)h / 2.0);
+  const float zh = zeta * halfh;
+  const float hz = h * zeta;
+  const float hz2 = hz * hz;
+  a[0] = (float)((double)h * (1.0 - (double)zh));
+  a[1] = h * halfh;
+  a[2] = (float)((1.0 - (double)hz / 2.0 + (double)hz2 / 4.0) / (double)h);
+  a[3] = (float)((double)halfh * (1.0 - (double)hz / 2.0));
+  return SOP_OK;
+}
+
 void sop_gen_opts_default(sop_gen_opts *o) {
   o->bond = 3.8f;
   o->min_dist = 4.0f;

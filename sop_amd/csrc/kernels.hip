@@ -969,10 +969,29 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
           p.fout[i] = make_float4(ftx, fty, ftz, 0.0f);
         } else {
           float4 x = partner<LDS>(pin, s_pos, bead_off, i);
-          const float a5 = S.a5;
-          x.x = x.x + a5 * ftx;
-          x.y = x.y + a5 * fty;
-          x.z = x.z + a5 * ftz;
+          if (S.dyn == 0) {
+            // overdamped (sop.cu:487-520): x += (h/zeta) F
+            const float a5 = S.a5;
+            x.x = x.x + a5 * ftx;
+            x.y = x.y + a5 * fty;
+            x.z = x.z + a5 * ftz;
+          } else {
+            // underdamped (sop.cu:421-484): the reference's velocity update
+            // after its force evaluation, v = a3 incr + a4 F, happens here at
+            // the start of the next step, then incr = a1 v + a2 F, x += incr
+            const float4 st = p.vel[i];
+            float vx = st.x, vy = st.y, vz = st.z;
+            if (__float_as_int(st.w) == 1) {
+              vx = S.a3 * st.x + S.a4 * ftx;
+              vy = S.a3 * st.y + S.a4 * fty;
+              vz = S.a3 * st.z + S.a4 * ftz;
+            }
+            const float ix = S.a1 * vx + S.a2 * ftx, iy = S.a1 * vy + S.a2 * fty, iz = S.a1 * vz + S.a2 * ftz;
+            x.x = x.x + ix;
+            x.y = x.y + iy;
+            x.z = x.z + iz;
+            p.vel[i] = S.vupd ? make_float4(ix, iy, iz, __int_as_float(1)) : make_float4(vx, vy, vz, 0.0f);
+          }
           pout[i] = x;
           if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
         }

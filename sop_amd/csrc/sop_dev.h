@@ -20,7 +20,7 @@ constexpr int ERR_BOND = 16;     // FENE bond stretched to |r - r0| >= R0
 constexpr int ERR_INTERNAL = 32; // cell counts inconsistent
 
 // kernel modes of the fused force kernel
-constexpr int MODE_STEP = 0;        // forces + random force + overdamped move
+constexpr int MODE_STEP = 0;        // forces + random force + move (overdamped or underdamped)
 constexpr int MODE_FORCE = 1;       // deterministic forces -> fout
 constexpr int MODE_FORCE_NOISE = 2; // forces + random force -> fout
 
@@ -41,7 +41,10 @@ struct SysDev {
   int64_t step_end;  // the step kernel moves the system only while step < step_end
   int32_t n_nl, n_pl;
   // dynamics
+  int32_t dyn;       // 0 overdamped, 1 underdamped
+  int32_t vupd;      // underdamped: velocities are updated (T >= 1e-5)
   float a5, var;     // h/zeta and sqrt(2 T zeta / h)
+  float a1, a2, a3, a4; // underdamped coefficients
   float boxl, inv_boxl;
   uint32_t seed_lo, seed_hi;
   // force field (type-pair tables indexed (ti-1)*2 + (tj-1))
@@ -98,6 +101,8 @@ struct DevPtrs {
   int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list
   int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
+  float4 *vel;          // underdamped state per bead: w = 1: xyz = last increment;
+                        // w = 0: xyz = velocity entering the next step
   float4 *fout;         // forces (force-eval modes)
   double *epart;        // per bead: 6 energy partials
   double *sysout;       // per system slot: SYSOUT_N reduced values

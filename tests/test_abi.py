@@ -8,6 +8,7 @@
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -40,11 +41,25 @@ def test_ctypes_table_covers_the_header():
     assert set(declared_functions()) == set(_abi.SIGNATURES)
 
 
-def test_struct_sizes_match_c_layout():
-    # offsets follow the C declaration order with natural alignment
-    assert C.sizeof(_abi.Params) == 200
-    assert _abi.Params.nl_factor.offset == 136 and _abi.Params.seed.offset == 192
-    assert C.sizeof(_abi.SystemDesc) == 8 * 9 + 200 + 8
+def test_struct_sizes_match_c_layout(tmp_path):
+    # the C compiler's sizes and offsets for include/sop.h against the ctypes mirror
+    src = os.path.join(ROOT, "tests", "helpers", "abi_layout.c")
+    exe = str(tmp_path / "abi_layout")
+    subprocess.run(["gcc", "-std=c11", "-o", exe, src], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    structs = {"sop_params": _abi.Params, "sop_system_desc": _abi.SystemDesc, "sop_engine_opts": _abi.EngineOpts,
+               "sop_serve_opts": _abi.ServeOpts, "sop_job_result": _abi.JobResult,
+               "sop_list_stats": _abi.ListStats, "sop_gen_opts": _abi.GenOpts}
+    checked = 0
+    for line in filter(None, out):
+        name, val = line.split()
+        if "." in name:
+            st, fld = name.split(".")
+            assert getattr(structs[st], fld).offset == int(val), name
+        else:
+            assert C.sizeof(structs[name]) == int(val), name
+        checked += 1
+    assert checked >= 20
 
 
 def test_engine_without_device_fails_loudly():
