@@ -190,6 +190,10 @@ sop_status sop_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nste
 sop_status sop_run_plain(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps);
 sop_status sop_get_coords(sop_engine *e, int32_t sys, float *xyz, int64_t *step);
 sop_status sop_set_coords(sop_engine *e, int32_t sys, const float *xyz);
+/* Philox key and the step counter of the next step (setup_rng,
+ * GPUvars.cu:430-453; the counter also sets the list schedule).  With the
+ * coordinates (and velocities) it is all a restart needs. */
+sop_status sop_set_rng(sop_engine *e, int32_t sys, uint64_t seed, int64_t step);
 /* Wrapped coordinates pos = unc - boxl*rint(unc/boxl) (sop.cu:510-512). */
 sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz);
 /* Velocities as the reference holds them after its last iteration (vel,

@@ -299,6 +299,10 @@ class Engine:
         self._c(lib().sop_get_wrapped_coords(self._h, sys, _ptr(x, C.c_float)))
         return x
 
+    def set_rng(self, sys, seed, step):
+        """Philox key and step counter of the next step (an exact restart)."""
+        self._c(lib().sop_set_rng(self._h, sys, seed, step))
+
     def velocities(self, sys) -> np.ndarray:
         v = np.zeros((self._n[sys], 3), np.float32)
         self._c(lib().sop_get_velocities(self._h, sys, _ptr(v, C.c_float)))

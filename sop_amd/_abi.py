@@ -123,6 +123,7 @@ SIGNATURES = {
     "sop_set_coords": (i32, [vp, i32, P(f32)]),
     "sop_get_wrapped_coords": (i32, [vp, i32, P(f32)]),
     "sop_get_velocities": (i32, [vp, i32, P(f32)]),
+    "sop_set_rng": (i32, [vp, i32, C.c_uint64, C.c_int64]),
     "sop_get_increments": (i32, [vp, i32, P(f32)]),
     "sop_list_stats_get": (i32, [vp, i32, P(ListStats)]),
     "sop_time_step_kernel": (i32, [vp, P(i32), i32, i32, P(f64)]),

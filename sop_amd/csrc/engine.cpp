@@ -1235,6 +1235,23 @@ sop_status sop_get_increments(sop_engine *e, int32_t sys, float *d3n) {
   return SOP_OK;
 }
 
+sop_status sop_set_rng(sop_engine *e, int32_t sys, uint64_t seed, int64_t step) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  if (step < 0) return set_error(e, SOP_ERR_INVALID, "step must be >= 0");
+  SysDev S;
+  sop_status st = engine_read_sys(e, sys, &S);
+  if (st) return st;
+  S.seed_lo = (uint32_t)seed;
+  S.seed_hi = (uint32_t)(seed >> 32);
+  S.step = step;
+  S.step_end = step;
+  HIPCK(hipMemcpy(e->p.sys + sys, &S, sizeof(SysDev), hipMemcpyHostToDevice));
+  SysHost &h = e->sys[sys];
+  h.params.seed = seed;
+  h.step = step;
+  return SOP_OK;
+}
+
 sop_status sop_set_coords(sop_engine *e, int32_t sys, const float *xyz) {
   if (!valid_sys(e, sys) || !xyz) return set_error(e, SOP_ERR_INVALID, "bad arguments");
   SysDev S;

@@ -14,19 +14,16 @@ import numpy as np
 import pytest
 
 import sop_amd as S
+from reffiles import write_reference_files
 from sop_amd import _abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = set()
-    for h in os.listdir(os.path.join(ROOT, "include")):
-        if h.endswith(".h"):
-            src = open(os.path.join(ROOT, "include", h)).read()
-            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-            names |= set(re.findall(r"\b(sop_[a-z0-9_]+)\s*\(", src))
-    return sorted(names)
+def declared_functions(header="sop.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(sop_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_every_declared_symbol_is_exported():
@@ -118,33 +115,6 @@ def test_native_contacts_match_brute_force():
         want = set(zip(i[ok].tolist(), j[ok].tolist()))
         assert set(zip(ni.tolist(), nj.tolist())) == want
         assert np.allclose(nr, d[ni, nj].astype(np.float32))
-
-
-def write_reference_files(tmp, sysd):
-    """Writes bonds/angles/vdw/init in the reference formats (io.cpp:112-259)."""
-    n = sysd.n
-    with open(os.path.join(tmp, "bonds.dat"), "w") as f:
-        f.write(f"nbnd {n - 1}\n")
-        for i in range(n - 1):
-            f.write(f"{i + 1} {i + 2} {sysd.bond_r0[i]:.6f}\n")
-    with open(os.path.join(tmp, "angles.dat"), "w") as f:
-        f.write(f"nang {n - 2}\n")
-        for i in range(n - 2):
-            f.write(f"{i + 1} {i + 2} {i + 3} 0.0\n")
-    nat = {(int(a), int(b)): float(r) for a, b, r in zip(sysd.nat_i, sysd.nat_j, sysd.nat_r0)}
-    x = sysd.xyz.astype(np.float64)
-    lines = []
-    for i in range(n):
-        for j in range(i + 3, n):
-            r = nat.get((i, j), max(8.5, float(np.linalg.norm(x[i] - x[j]))))
-            lines.append(f"{i + 1} {j + 1} {r:.6f} 1 1\n")
-    with open(os.path.join(tmp, "vdw.dat"), "w") as f:
-        f.write(f"ncon_att {len(nat)} ncon_rep {len(lines) - len(nat)}\n")
-        f.writelines(lines)
-    with open(os.path.join(tmp, "init.dat"), "w") as f:
-        f.write(f"nbead {n}\n")
-        for i in range(n):
-            f.write(f"{i + 1} {x[i, 0]:.6f} {x[i, 1]:.6f} {x[i, 2]:.6f}\n")
 
 
 def test_reference_input_files_round_trip(tmp_path):
