@@ -154,9 +154,10 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ImportError(f"libsop.so not built at {LIB_PATH}: run __graft_entry__.build()")
-    lib = C.CDLL(LIB_PATH)
+    path = os.environ.get("SOP_LIBSOP", LIB_PATH)  # another build of libsop.so, for comparisons
+    if not os.path.exists(path):
+        raise ImportError(f"libsop.so not built at {path}: run __graft_entry__.build()")
+    lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

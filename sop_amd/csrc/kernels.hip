@@ -268,11 +268,24 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
         float a = red[d][0], b = red[3 + d][0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { a = fminf(a, red[d][w]); b = fmaxf(b, red[3 + d][w]); }
         const float ext = b - a;
+        // Cells of the minimum edge, one more than fit in the extent so the
+        // grid covers it; when that exceeds dmax, dmax cells stretched over
+        // the extent (edge ext/dmax >= min_edge).  Stretching every cell to
+        // ext/floor(ext/min_edge) instead would make the 27-cell stencil up
+        // to 8x the volume it needs to be.
         int dims = 1;
-        if (ext > 0.0f) dims = (int)floorf(ext / min_edge);
-        dims = min(max(dims, 1), S.dmax);
+        float inv = 0.0f;
+        if (ext > 0.0f) {
+          dims = (int)floorf(ext / min_edge) + 1;
+          if (dims <= S.dmax) {
+            inv = 1.0f / min_edge;
+          } else {
+            dims = S.dmax;
+            inv = (float)dims / ext;
+          }
+        }
         S.org[d] = a;
-        S.inv_edge[d] = ext > 0.0f ? (float)dims / ext : 0.0f;
+        S.inv_edge[d] = inv;
         S.dims[d] = dims;
         nc *= dims;
         ext_max = fmaxf(ext_max, ext);
@@ -385,6 +398,7 @@ __global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep) {
         for (int m = st; m < en; ++m) rank += p.cell_tmp[S.bead_off + m] < i;
         const int pos = st + rank;
         if (pos < S.n) {
+          p.cell_pos[i] = pos;
           p.cell_sorted[S.bead_off + pos] = i;
           p.pos_sorted[S.bead_off + pos] = build_src(p, S, kstep)[i];
         }
@@ -523,14 +537,69 @@ __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
 // of the cell tests them.  Each bead's list comes out in row order, then
 // cell-sorted order: the order of a per-bead scan of the same cells.
 constexpr int HOME_MAX = 64;   // home beads per pass
-constexpr int HOME_NAT = 32;   // natives staged per home bead (more: exact global search)
-constexpr int CAND_MAX = 1024; // candidates staged per pass
+constexpr int CAND_MAX = 1024; // candidates staged per pass (and bits of the exclusion bitmaps)
+constexpr int EXCL_WORDS = CAND_MAX / 32;
+
+// One home bead against the nc staged candidates [b0, b0 + nc), two per lane
+// per pass with unconditional (clamped) LDS reads and no branch in the loop.
+// Native partners are excluded exactly: by the home bead's bitmap over
+// candidate slots, or (exact = true, more candidates than the bitmap holds)
+// by a search of its sorted native partners.  Appends the partners within the
+// pair cutoff to out in candidate order; returns the new count.
+template <bool PBC>
+__device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_cj, int nc, int b0, float4 xi, int ji,
+                                            int chi, float rcA, float rcB, float boxl, const uint32_t *excl,
+                                            bool exact, const int32_t *nat_g, int nn, int32_t *out, int cnt,
+                                            int pl_cap, int lane) {
+  for (int k0 = 0; k0 < nc; k0 += 128) {
+    int jj[2];
+    float4 xj[2];
+    uint32_t ew[2];
+    bool pr[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int ci = min(k0 + u * 64 + lane, nc - 1);
+      jj[u] = s_cj[ci];
+      xj[u] = s_cand[ci];
+      ew[u] = exact ? 0u : excl[(b0 + ci) >> 5] >> ((b0 + ci) & 31);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = jj[u];
+      const int wj = __float_as_int(xj[u].w);
+      const bool skip = (j == ji) | (((wj >> 2) == chi) & ((unsigned)(j - ji + 2) <= 4u)) | ((ew[u] & 1u) != 0u);
+      float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+      if (PBC) {
+        dx = dx - boxl * rintf(dx / boxl);
+        dy = dy - boxl * rintf(dy / boxl);
+        dz = dz - boxl * rintf(dz / boxl);
+      }
+      const float d2 = (dx * dx + dy * dy) + dz * dz;
+      pr[u] = (k0 + u * 64 + lane < nc) & !skip & (d2 < ((wj & 3) == 2 ? rcB : rcA));
+    }
+    if (exact) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (pr[u] && native_pair(nat_g, nn, jj[u])) pr[u] = false;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint64_t mk = __ballot(pr[u]);
+      if (pr[u]) {
+        const int o = cnt + lanes_below(mk);
+        if (o < pl_cap) out[o] = jj[u];
+      }
+      cnt += __popcll(mk);
+    }
+  }
+  return cnt;
+}
+
 
 __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
   __shared__ float4 s_home[HOME_MAX];
   __shared__ int s_hj[HOME_MAX], s_hcnt[HOME_MAX], s_hnn[HOME_MAX], s_hns[HOME_MAX];
-  __shared__ int s_nat[HOME_MAX][HOME_NAT];
-  __shared__ uint32_t s_mask[HOME_MAX][8];
+  __shared__ uint32_t s_excl[HOME_MAX][EXCL_WORDS];  // per home bead: candidate slots of its natives
   __shared__ float4 s_cand[CAND_MAX];
   __shared__ int s_cj[CAND_MAX];
   __shared__ int s_rst[9], s_roff[10];
@@ -570,6 +639,7 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
     }
     __syncthreads();
     const int T = s_roff[9];
+    const bool exact = T > CAND_MAX;
     for (int h0 = hs; h0 < he; h0 += HOME_MAX) {
       const int nh = min(HOME_MAX, he - h0);
       if (tid < nh) {
@@ -580,16 +650,24 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         s_hns[tid] = p.nat_start[j];
         s_hnn[tid] = p.nat_cnt[j];
 #pragma unroll
-        for (int w = 0; w < 8; ++w) s_mask[tid][w] = 0u;
+        for (int w = 0; w < EXCL_WORDS; ++w) s_excl[tid][w] = 0u;
       }
       __syncthreads();
-      for (int h = wave; h < nh; h += 4) {
-        const int ns = s_hns[h], nn = s_hnn[h];
-        for (int k = lane; k < nn; k += 64) {
-          const int jn = p.nat_j[ns + k];
-          if (k < HOME_NAT) s_nat[h][k] = jn;
-          const int key = (jn - bo) & 255;
-          atomicOr(&s_mask[h][key >> 5], 1u << (key & 31));
+      if (!exact) {
+        // slot of each native partner among the candidates: its cell (this
+        // rebuild's grid) picks the row, its sorted position the offset
+        for (int h = wave; h < nh; h += 4) {
+          const int ns = s_hns[h], nn = s_hnn[h];
+          for (int k = lane; k < nn; k += 64) {
+            const int jn = p.nat_j[ns + k];
+            const int cn = p.cell_of[jn], pos = p.cell_pos[jn];
+            const int nx = cn % d0, ny = (cn / d0) % d1, nz = cn / (d0 * d1);
+            if (abs(nx - cx) <= 1 && abs(ny - cy) <= 1 && abs(nz - cz) <= 1) {
+              const int r = (nz - cz + 1) * 3 + (ny - cy + 1);
+              const int slot = s_roff[r] + (pos - s_rst[r]);
+              if (slot >= s_roff[r] && slot < s_roff[r + 1]) atomicOr(&s_excl[h][slot >> 5], 1u << (slot & 31));
+            }
+          }
         }
       }
       for (int b0 = 0; b0 < T; b0 += CAND_MAX) {
@@ -609,41 +687,13 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           const int ji = s_hj[h];
           const int wi = __float_as_int(xi.w), chi = wi >> 2;
           const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
-          const int nn = s_hnn[h];
-          int cnt = s_hcnt[h];
           int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
-          for (int k0 = 0; k0 < nc; k0 += 64) {
-            const int ci = k0 + lane;
-            bool pred = false;
-            int j = 0;
-            if (ci < nc) {
-              j = s_cj[ci];
-              const float4 xj = s_cand[ci];
-              const int wj = __float_as_int(xj.w);
-              const int dj = j - ji;
-              const bool excl = (j == ji) || (((wj >> 2) == chi) && dj >= -2 && dj <= 2);
-              if (!excl && dist2_exact(xi, xj, boxl) < ((wj & 3) == 2 ? rcB : rcA)) {
-                const int key = (j - bo) & 255;
-                bool nat = (s_mask[h][key >> 5] >> (key & 31)) & 1u;
-                if (nat) {
-                  if (nn <= HOME_NAT) {
-                    bool hit = false;
-                    for (int q = 0; q < nn; ++q) hit |= s_nat[h][q] == j;
-                    nat = hit;
-                  } else {
-                    nat = native_pair(p.nat_j + s_hns[h], nn, j);
-                  }
-                }
-                pred = !nat;
-              }
-            }
-            const uint64_t mk = __ballot(pred);
-            if (pred) {
-              const int o = cnt + lanes_below(mk);
-              if (o < pl_cap) out[o] = j;
-            }
-            cnt += __popcll(mk);
-          }
+          const int32_t *natg = p.nat_j + s_hns[h];
+          const int cnt = boxl > 0.0f
+              ? pl_scan_home<true>(s_cand, s_cj, nc, b0, xi, ji, chi, rcA, rcB, boxl, s_excl[h], exact, natg,
+                                   s_hnn[h], out, s_hcnt[h], pl_cap, lane)
+              : pl_scan_home<false>(s_cand, s_cj, nc, b0, xi, ji, chi, rcA, rcB, boxl, s_excl[h], exact, natg,
+                                    s_hnn[h], out, s_hcnt[h], pl_cap, lane);
           if (lane == 0) s_hcnt[h] = cnt;
         }
         __syncthreads();

@@ -101,6 +101,7 @@ struct DevPtrs {
   int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list
   int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
+  int32_t *cell_pos;    // per bead: its position in the system's cell-sorted order
   float4 *vel;          // underdamped state per bead: w = 1: xyz = last increment;
                         // w = 0: xyz = velocity entering the next step
   float4 *fout;         // forces (force-eval modes)
