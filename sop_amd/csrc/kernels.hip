@@ -541,16 +541,19 @@ constexpr int CAND_MAX = 1024; // candidates staged per pass (and bits of the ex
 constexpr int EXCL_WORDS = CAND_MAX / 32;
 
 // One home bead against the nc staged candidates [b0, b0 + nc), two per lane
-// per pass with unconditional (clamped) LDS reads and no branch in the loop.
-// Native partners are excluded exactly: by the home bead's bitmap over
-// candidate slots, or (exact = true, more candidates than the bitmap holds)
-// by a search of its sorted native partners.  Appends the partners within the
-// pair cutoff to out in candidate order; returns the new count.
-template <bool PBC>
-__device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_cj, int nc, int b0, float4 xi, int ji,
+// per pass with no branch in the loop.  Fast form (EXACT = false): the staged
+// arrays are padded to a multiple of 128 with unreachable coordinates, and the
+// home bead's bitmap over candidate slots already excludes itself, its chain
+// neighbours i +- 1, 2 and its native partners, so a candidate costs its loads,
+// the distance and one bit test.  EXACT = true (more candidates than the
+// bitmap holds): bounds, topology and a search of the sorted native partners
+// per candidate.  Appends the partners within the pair cutoff to out in
+// candidate order; returns the new count.
+template <bool PBC, bool EXACT>
+__device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_cj, int nc, float4 xi, int ji,
                                             int chi, float rcA, float rcB, float boxl, const uint32_t *excl,
-                                            bool exact, const int32_t *nat_g, int nn, int32_t *out, int cnt,
-                                            int pl_cap, int lane) {
+                                            const int32_t *nat_g, int nn, int32_t *out, int cnt, int pl_cap,
+                                            int lane) {
   for (int k0 = 0; k0 < nc; k0 += 128) {
     int jj[2];
     float4 xj[2];
@@ -558,16 +561,15 @@ __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_c
     bool pr[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int ci = min(k0 + u * 64 + lane, nc - 1);
+      const int ci = EXACT ? min(k0 + u * 64 + lane, nc - 1) : k0 + u * 64 + lane;
       jj[u] = s_cj[ci];
       xj[u] = s_cand[ci];
-      ew[u] = exact ? 0u : excl[(b0 + ci) >> 5] >> ((b0 + ci) & 31);
+      ew[u] = EXACT ? 0u : excl[ci >> 5] >> (ci & 31);
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int j = jj[u];
       const int wj = __float_as_int(xj[u].w);
-      const bool skip = (j == ji) | (((wj >> 2) == chi) & ((unsigned)(j - ji + 2) <= 4u)) | ((ew[u] & 1u) != 0u);
       float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
       if (PBC) {
         dx = dx - boxl * rintf(dx / boxl);
@@ -575,9 +577,15 @@ __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_c
         dz = dz - boxl * rintf(dz / boxl);
       }
       const float d2 = (dx * dx + dy * dy) + dz * dz;
-      pr[u] = (k0 + u * 64 + lane < nc) & !skip & (d2 < ((wj & 3) == 2 ? rcB : rcA));
+      bool keep = d2 < ((wj & 3) == 2 ? rcB : rcA);
+      if (EXACT) {
+        keep &= (k0 + u * 64 + lane < nc) & (j != ji) & !(((wj >> 2) == chi) & ((unsigned)(j - ji + 2) <= 4u));
+      } else {
+        keep &= (ew[u] & 1u) == 0u;
+      }
+      pr[u] = keep;
     }
-    if (exact) {
+    if (EXACT) {
 #pragma unroll
       for (int u = 0; u < 2; ++u)
         if (pr[u] && native_pair(nat_g, nn, jj[u])) pr[u] = false;
@@ -640,6 +648,26 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
     __syncthreads();
     const int T = s_roff[9];
     const bool exact = T > CAND_MAX;
+    // stage the candidates [b0, b0 + nc); the fast form pads to 128 with
+    // coordinates no pair can reach
+    auto stage = [&](int b0, int nc) {
+      const int ncp = exact ? nc : (nc + 127) & ~127;
+      for (int ci = tid; ci < ncp; ci += 256) {
+        const int idx = b0 + ci;
+        if (ci < nc) {
+          int r = 0;
+#pragma unroll
+          for (int q = 1; q < 9; ++q) r += s_roff[q] <= idx;
+          const int m = s_rst[r] + (idx - s_roff[r]);
+          s_cand[ci] = p.pos_sorted[bo + m];
+          s_cj[ci] = p.cell_sorted[bo + m];
+        } else {
+          s_cand[ci] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+          s_cj[ci] = -1;
+        }
+      }
+    };
+    if (!exact) stage(0, T);
     for (int h0 = hs; h0 < he; h0 += HOME_MAX) {
       const int nh = min(HOME_MAX, he - h0);
       if (tid < nh) {
@@ -654,32 +682,24 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
       }
       __syncthreads();
       if (!exact) {
-        // slot of each native partner among the candidates: its cell (this
-        // rebuild's grid) picks the row, its sorted position the offset
+        // exclusion bitmaps: the slot of each native partner and of the bead
+        // itself and its chain neighbours i +- 1, 2, from its cell in this
+        // rebuild's grid (the row) and its sorted position (the offset)
         for (int h = wave; h < nh; h += 4) {
-          const int ns = s_hns[h], nn = s_hnn[h];
-          for (int k = lane; k < nn; k += 64) {
-            const int jn = p.nat_j[ns + k];
+          const int ji = s_hj[h], ns = s_hns[h], nn = s_hnn[h];
+          const int chi = __float_as_int(s_home[h].w) >> 2;
+          for (int k = lane; k < nn + 5; k += 64) {
+            const int jn = k < nn ? p.nat_j[ns + k] : ji + (k - nn - 2);
+            if (jn < bo || jn >= bo + n) continue;
             const int cn = p.cell_of[jn], pos = p.cell_pos[jn];
             const int nx = cn % d0, ny = (cn / d0) % d1, nz = cn / (d0 * d1);
-            if (abs(nx - cx) <= 1 && abs(ny - cy) <= 1 && abs(nz - cz) <= 1) {
-              const int r = (nz - cz + 1) * 3 + (ny - cy + 1);
-              const int slot = s_roff[r] + (pos - s_rst[r]);
-              if (slot >= s_roff[r] && slot < s_roff[r + 1]) atomicOr(&s_excl[h][slot >> 5], 1u << (slot & 31));
-            }
+            if (abs(nx - cx) > 1 || abs(ny - cy) > 1 || abs(nz - cz) > 1) continue;
+            const int r = (nz - cz + 1) * 3 + (ny - cy + 1);
+            const int slot = s_roff[r] + (pos - s_rst[r]);
+            if (slot < s_roff[r] || slot >= s_roff[r + 1] || s_cj[slot] != jn) continue;
+            if (k >= nn && (__float_as_int(s_cand[slot].w) >> 2) != chi) continue;  // another chain
+            atomicOr(&s_excl[h][slot >> 5], 1u << (slot & 31));
           }
-        }
-      }
-      for (int b0 = 0; b0 < T; b0 += CAND_MAX) {
-        const int nc = min(CAND_MAX, T - b0);
-        for (int ci = tid; ci < nc; ci += 256) {
-          const int idx = b0 + ci;
-          int r = 0;
-#pragma unroll
-          for (int q = 1; q < 9; ++q) r += s_roff[q] <= idx;
-          const int m = s_rst[r] + (idx - s_roff[r]);
-          s_cand[ci] = p.pos_sorted[bo + m];
-          s_cj[ci] = p.cell_sorted[bo + m];
         }
         __syncthreads();
         for (int h = wave; h < nh; h += 4) {
@@ -688,16 +708,36 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           const int wi = __float_as_int(xi.w), chi = wi >> 2;
           const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
           int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
-          const int32_t *natg = p.nat_j + s_hns[h];
           const int cnt = boxl > 0.0f
-              ? pl_scan_home<true>(s_cand, s_cj, nc, b0, xi, ji, chi, rcA, rcB, boxl, s_excl[h], exact, natg,
-                                   s_hnn[h], out, s_hcnt[h], pl_cap, lane)
-              : pl_scan_home<false>(s_cand, s_cj, nc, b0, xi, ji, chi, rcA, rcB, boxl, s_excl[h], exact, natg,
-                                    s_hnn[h], out, s_hcnt[h], pl_cap, lane);
+              ? pl_scan_home<true, false>(s_cand, s_cj, T, xi, ji, chi, rcA, rcB, boxl, s_excl[h], nullptr, 0, out,
+                                          0, pl_cap, lane)
+              : pl_scan_home<false, false>(s_cand, s_cj, T, xi, ji, chi, rcA, rcB, boxl, s_excl[h], nullptr, 0, out,
+                                           0, pl_cap, lane);
           if (lane == 0) s_hcnt[h] = cnt;
         }
-        __syncthreads();
+      } else {
+        for (int b0 = 0; b0 < T; b0 += CAND_MAX) {
+          const int nc = min(CAND_MAX, T - b0);
+          stage(b0, nc);
+          __syncthreads();
+          for (int h = wave; h < nh; h += 4) {
+            const float4 xi = s_home[h];
+            const int ji = s_hj[h];
+            const int wi = __float_as_int(xi.w), chi = wi >> 2;
+            const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
+            int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
+            const int32_t *natg = p.nat_j + s_hns[h];
+            const int cnt = boxl > 0.0f
+                ? pl_scan_home<true, true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, nullptr, natg, s_hnn[h],
+                                           out, s_hcnt[h], pl_cap, lane)
+                : pl_scan_home<false, true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, nullptr, natg, s_hnn[h],
+                                            out, s_hcnt[h], pl_cap, lane);
+            if (lane == 0) s_hcnt[h] = cnt;
+          }
+          __syncthreads();
+        }
       }
+      __syncthreads();
       // counts, and the native pair lists of the home beads
       if (tid < nh) {
         const int j = s_hj[tid], cr = s_hcnt[tid];
