@@ -408,6 +408,7 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   S.n = n;
   S.nl_cap = nl_cap;
   S.pl_cap = pl_cap;
+  S.idx16 = n <= 65536 ? 1 : 0;
   S.nl_base = h.nl_off;
   S.pl_base = h.pl_off;
   S.cell_base = (int32_t)h.cell_off;
@@ -916,8 +917,15 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
     cnt_d = nl ? e->p.nlrep_cnt : e->p.plrep_cnt;
     seg_len = nl ? h.nl_len : h.pl_len;
     ent.resize((size_t)seg_len);
-    HIPCK(hipMemcpy(ent.data(), (nl ? e->p.nlrep_j + h.nl_off : e->p.plrep_j + h.pl_off), 4 * (size_t)seg_len,
-                    hipMemcpyDeviceToHost));
+    if (!nl && h.n <= 65536) {
+      // 16-bit local indices, rows of pl_cap entries packed from the segment start
+      std::vector<uint16_t> e16((size_t)seg_len);
+      HIPCK(hipMemcpy(e16.data(), e->p.plrep_j + h.pl_off, 2 * (size_t)seg_len, hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < seg_len; ++q) ent[(size_t)q] = (int32_t)h.bead_off + e16[(size_t)q];
+    } else {
+      HIPCK(hipMemcpy(ent.data(), (nl ? e->p.nlrep_j + h.nl_off : e->p.plrep_j + h.pl_off), 4 * (size_t)seg_len,
+                      hipMemcpyDeviceToHost));
+    }
     const int32_t capb = nl ? h.nl_cap : h.pl_cap;
     for (int i = 0; i < nb; ++i) start[i] = i * capb;
   } else {

@@ -552,8 +552,8 @@ constexpr int EXCL_WORDS = CAND_MAX / 32;
 template <bool PBC, bool EXACT>
 __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_cj, int nc, float4 xi, int ji,
                                             int chi, float rcA, float rcB, float boxl, const uint32_t *excl,
-                                            const int32_t *nat_g, int nn, int32_t *out, int cnt, int pl_cap,
-                                            int lane) {
+                                            const int32_t *nat_g, int nn, int32_t *out, uint16_t *out16, int bo,
+                                            int cnt, int pl_cap, int lane) {
   for (int k0 = 0; k0 < nc; k0 += 128) {
     int jj[2];
     float4 xj[2];
@@ -595,7 +595,10 @@ __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_c
       const uint64_t mk = __ballot(pr[u]);
       if (pr[u]) {
         const int o = cnt + lanes_below(mk);
-        if (o < pl_cap) out[o] = jj[u];
+        if (o < pl_cap) {
+          if (out16) out16[o] = (uint16_t)(jj[u] - bo);
+          else out[o] = jj[u];
+        }
       }
       cnt += __popcll(mk);
     }
@@ -708,11 +711,12 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           const int wi = __float_as_int(xi.w), chi = wi >> 2;
           const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
           int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
+          uint16_t *out16 = S.idx16 ? (uint16_t *)(p.plrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
           const int cnt = boxl > 0.0f
               ? pl_scan_home<true, false>(s_cand, s_cj, T, xi, ji, chi, rcA, rcB, boxl, s_excl[h], nullptr, 0, out,
-                                          0, pl_cap, lane)
+                                          out16, bo, 0, pl_cap, lane)
               : pl_scan_home<false, false>(s_cand, s_cj, T, xi, ji, chi, rcA, rcB, boxl, s_excl[h], nullptr, 0, out,
-                                           0, pl_cap, lane);
+                                           out16, bo, 0, pl_cap, lane);
           if (lane == 0) s_hcnt[h] = cnt;
         }
       } else {
@@ -726,12 +730,13 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
             const int wi = __float_as_int(xi.w), chi = wi >> 2;
             const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
             int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
+            uint16_t *out16 = S.idx16 ? (uint16_t *)(p.plrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
             const int32_t *natg = p.nat_j + s_hns[h];
             const int cnt = boxl > 0.0f
                 ? pl_scan_home<true, true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, nullptr, natg, s_hnn[h],
-                                           out, s_hcnt[h], pl_cap, lane)
+                                           out, out16, bo, s_hcnt[h], pl_cap, lane)
                 : pl_scan_home<false, true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, nullptr, natg, s_hnn[h],
-                                            out, s_hcnt[h], pl_cap, lane);
+                                            out, out16, bo, s_hcnt[h], pl_cap, lane);
             if (lane == 0) s_hcnt[h] = cnt;
           }
           __syncthreads();
@@ -889,7 +894,10 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
         const uint64_t mk = __ballot(pred);
         if (pred) {
           const int o = cr + lanes_below(mk);
-          if (o < S.pl_cap) p.plrep_j[pb + o] = j;
+          if (o < S.pl_cap) {
+            if (S.idx16) ((uint16_t *)(p.plrep_j + S.pl_base))[pb - S.pl_base + o] = (uint16_t)(j - S.bead_off);
+            else p.plrep_j[pb + o] = j;
+          }
         }
         cr += __popcll(mk);
       }
@@ -914,7 +922,7 @@ __device__ __forceinline__ float4 partner(const float4 *__restrict__ pin, const 
 // reduce; a 256-thread block covers the tile in LPB/4 rounds, stages the 64
 // forces in LDS, and wave 0 draws the random force and moves the 64 beads
 // (one bead per lane).
-template <int LPB, int MODE, bool LDS>
+template <int LPB, int MODE, bool LDS, bool IDX16>
 __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int s, int t, int kstep,
                                           const float4 *__restrict__ pin, float4 *__restrict__ pout,
                                           float4 *sF, const float4 *s_pos) {
@@ -1006,6 +1014,8 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
         const float m6A = S.rep_m6[row], m6B = S.rep_m6[row + 1];
         const float gA = S.rep_guard[row], gB = S.rep_guard[row + 1];
         const int32_t *__restrict__ lst = p.plrep_j + pb;
+        const uint16_t *__restrict__ lst16 = (const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base);
+        auto ent = [&](int k) -> int { return IDX16 ? bead_off + (int)lst16[k] : lst[k]; };
         auto rep = [&](float4 xj) {
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           if (!wrapf) {
@@ -1027,7 +1037,7 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
         };
         int k = l;
         for (; k + 3 * LPB < nr; k += 4 * LPB) {
-          const int j0 = lst[k], j1 = lst[k + LPB], j2 = lst[k + 2 * LPB], j3 = lst[k + 3 * LPB];
+          const int j0 = ent(k), j1 = ent(k + LPB), j2 = ent(k + 2 * LPB), j3 = ent(k + 3 * LPB);
           const float4 x0 = partner<LDS>(pin, s_pos, bead_off, j0), x1 = partner<LDS>(pin, s_pos, bead_off, j1);
           const float4 x2 = partner<LDS>(pin, s_pos, bead_off, j2), x3 = partner<LDS>(pin, s_pos, bead_off, j3);
           rep(x0);
@@ -1035,7 +1045,7 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
           rep(x2);
           rep(x3);
         }
-        for (; k < nr; k += LPB) rep(partner<LDS>(pin, s_pos, bead_off, lst[k]));
+        for (; k < nr; k += LPB) rep(partner<LDS>(pin, s_pos, bead_off, ent(k)));
       }
     }
     fx = group_sum<LPB>(fx);
@@ -1117,9 +1127,15 @@ __global__ __launch_bounds__(256) void k_step(DevPtrs p, int kstep) {
     if (it.w & ITEM_LDS) {
       for (int b = tid; b < S.n; b += blockDim.x) s_pos[b] = pin[S.bead_off + b];
       __syncthreads();
-      for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, true>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+      if (S.idx16)
+        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, true, true>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+      else
+        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, true, false>(p, S, s, t, kstep, pin, pout, sF, s_pos);
     } else {
-      for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, false>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+      if (S.idx16)
+        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, false, true>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+      else
+        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, false, false>(p, S, s, t, kstep, pin, pout, sF, s_pos);
     }
   }
 }
@@ -1214,7 +1230,8 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
         const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
         const int nr = p.plrep_cnt[i];
         for (int k = l; k < nr; k += LPB) {
-          const int j = p.plrep_j[pb + k];
+          const int j = S.idx16 ? bead_off + ((const uint16_t *)(p.plrep_j + S.pl_base))[pb - S.pl_base + k]
+                                : p.plrep_j[pb + k];
           const float4 xj = pin[j];
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           if (!wrapf) {

@@ -30,6 +30,7 @@ struct SysDev {
   int32_t n;         // beads
   int32_t nl_cap;    // per-bead capacity of the non-native neighbor list
   int32_t pl_cap;    // per-bead capacity of the non-native pair list
+  int32_t idx16;     // 1: pair-list rows hold 16-bit local indices (n <= 65536)
   int64_t nl_base;   // first entry of this system's neighbor-list segment
   int64_t pl_base;   // first entry of this system's pair-list segment
   int32_t cell_base; // first cell of this system's cell segment (cell_cap+1 slots)
@@ -98,7 +99,9 @@ struct DevPtrs {
   int32_t *nlatt_j; float *nlatt_r0; int32_t *nlatt_cnt; // native neighbor list (same slots)
   int32_t *platt_j; float *platt_r0; int32_t *platt_cnt; // native pair list (same slots)
   int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list
-  int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list
+  int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list: row i at pl_base +
+                                                         // (i - bead_off) * pl_cap, entries int32 arena
+                                                         // indices, or uint16 local indices (idx16)
   int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
   int32_t *cell_pos;    // per bead: its position in the system's cell-sorted order
