@@ -88,10 +88,18 @@ int grid_sys(const sop_engine *e) {
 }
 
 int grid_items(const sop_engine *e) {
-  int g = e->opts.grid_blocks > 0 ? e->opts.grid_blocks : 8 * e->num_cu;
+  int g = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
   g = std::max(8, g / 8 * 8);
   if (e->fixed_grids) return g;
   const int need = (e->cur_nitems + 7) / 8 * 8;
+  return std::max(8, std::min(g, need));
+}
+
+int grid_items_far(const sop_engine *e) {
+  int g = 2 * e->num_cu;
+  g = std::max(8, g / 8 * 8);
+  if (e->fixed_grids) return g;
+  const int need = (e->cur_nitems_far + 7) / 8 * 8;
   return std::max(8, std::min(g, need));
 }
 
@@ -548,34 +556,83 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   e->h_work.assign(mx * NX, -1);
   for (int x = 0; x < NX; ++x)
     for (size_t m = 0; m < bins[x].size(); ++m) e->h_work[m * NX + x] = bins[x][m];
-  // step-kernel items: runs of up to ITEM_TILES_MAX tiles of one system, on the
-  // same XCD bin as the system's tiles
-  std::vector<std::vector<int4>> ibins(NX);
+  // Step-kernel items of up to STEP_WAVES tiles (one per wave).  Systems that
+  // fit the LDS budget are staged whole: arena-adjacent small systems share an
+  // item (one staged range), a system of more than STEP_WAVES tiles gets one
+  // item per STEP_WAVES tiles, each staging the whole system.  Larger systems
+  // gather from global memory and keep all their items on one XCD.
+  const int64_t lds_cap = e->opts.lds_beads_max > 0
+                              ? std::min<int64_t>(e->opts.lds_beads_max, (int64_t)(e->lds_limit / 16))
+                              : 0;
+  std::vector<int32_t> by_arena(slots, slots + nslots);
+  std::sort(by_arena.begin(), by_arena.end(),
+            [&](int32_t a2, int32_t b2) { return e->sys[a2].bead_off < e->sys[b2].bead_off; });
+  std::vector<int4> lds_items;
+  std::vector<std::vector<int4>> far_items;  // per non-LDS system
+  int4 cur = make_int4(-1, 0, -1, 0);
+  auto flush = [&]() {
+    if (cur.x >= 0) lds_items.push_back(cur);
+    cur = make_int4(-1, 0, -1, 0);
+  };
   size_t lds = 0;
-  const int lds_max = e->opts.lds_beads_max;
-  for (int x = 0; x < NX; ++x) {
-    int32_t cur_sys = -1, first = 0, cnt = 0, flags = 0;
-    for (size_t m = 0; m < bins[x].size(); ++m) {
-      const int32_t t = bins[x][m], sys_of_t = bsys[x][m];
-      if (sys_of_t != cur_sys || cnt == ITEM_TILES_MAX || t != first + cnt) {
-        if (cur_sys >= 0) ibins[x].push_back(make_int4(cur_sys, first, cnt, flags));
-        cur_sys = sys_of_t;
-        first = t;
-        cnt = 0;
-        const int32_t n = e->sys[sys_of_t].n;
-        flags = (lds_max > 0 && n <= lds_max && (size_t)n * 16 <= e->lds_limit) ? ITEM_LDS : 0;
-        if (flags) lds = std::max(lds, (size_t)n * 16);
+  for (int32_t sl : by_arena) {
+    const SysHost &h = e->sys[sl];
+    const int32_t t0 = (int32_t)(h.bead_off / TILE), nt = (int32_t)(h.bead_len / TILE);
+    if (h.bead_len <= lds_cap) {
+      lds = std::max(lds, (size_t)h.bead_len * 16);
+      if (nt > STEP_WAVES) {
+        flush();
+        for (int32_t k = 0; k < nt; k += STEP_WAVES)
+          lds_items.push_back(make_int4(t0 + k, std::min(STEP_WAVES, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
+        continue;
       }
-      ++cnt;
+      if (cur.x >= 0 && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= STEP_WAVES &&
+          (int64_t)cur.w + h.bead_len <= lds_cap) {
+        cur.y += nt;
+        cur.w += (int32_t)h.bead_len;
+        lds = std::max(lds, (size_t)cur.w * 16);
+      } else {
+        flush();
+        cur = make_int4(t0, nt, (int32_t)h.bead_off, (int32_t)h.bead_len);
+      }
+    } else {
+      flush();
+      std::vector<int4> v;
+      for (int32_t k = 0; k < nt; k += STEP_WAVES) v.push_back(make_int4(t0 + k, std::min(STEP_WAVES, nt - k), -1, 0));
+      far_items.push_back(v);
     }
-    if (cur_sys >= 0) ibins[x].push_back(make_int4(cur_sys, first, cnt, flags));
   }
-  size_t imx = 0;
-  for (auto &b : ibins) imx = std::max(imx, b.size());
-  e->h_items.assign(imx * NX, make_int4(-1, 0, 0, 0));
-  for (int x = 0; x < NX; ++x)
+  flush();
+  // XCD bins by tile load (item k runs on XCD k mod 8): the LDS items, then
+  // whole non-LDS systems (largest first) so each keeps its gathers in one L2
+  std::vector<std::vector<int4>> ibins(NX), fbins(NX);
+  std::vector<int64_t> iload(NX, 0), fload(NX, 0);
+  std::stable_sort(lds_items.begin(), lds_items.end(), [](const int4 &u, const int4 &v) { return u.y > v.y; });
+  for (const int4 &it : lds_items) {
+    const int x = (int)(std::min_element(iload.begin(), iload.end()) - iload.begin());
+    ibins[x].push_back(it);
+    iload[x] += it.y;
+  }
+  std::sort(far_items.begin(), far_items.end(), [](const std::vector<int4> &u, const std::vector<int4> &v) {
+    return u.size() != v.size() ? u.size() > v.size() : u[0].x < v[0].x;
+  });
+  for (const auto &v : far_items) {
+    const int x = (int)(std::min_element(fload.begin(), fload.end()) - fload.begin());
+    for (const int4 &it : v) {
+      fbins[x].push_back(it);
+      fload[x] += it.y;
+    }
+  }
+  size_t imx = 0, fmx = 0;
+  for (auto &bn : ibins) imx = std::max(imx, bn.size());
+  for (auto &bn : fbins) fmx = std::max(fmx, bn.size());
+  e->h_items.assign((imx + fmx) * NX, make_int4(-1, 0, -1, 0));
+  for (int x = 0; x < NX; ++x) {
     for (size_t m = 0; m < ibins[x].size(); ++m) e->h_items[m * NX + x] = ibins[x][m];
-  if (e->fixed_grids && lds_max > 0) lds = std::min((size_t)lds_max * 16, e->lds_limit);
+    for (size_t m = 0; m < fbins[x].size(); ++m) e->h_items[(imx + m) * NX + x] = fbins[x][m];
+  }
+  const int32_t n_lds_items = (int32_t)(imx * NX), n_far_items = (int32_t)(fmx * NX);
+  if (e->fixed_grids && lds_cap > 0) lds = (size_t)lds_cap * 16;
   e->cur_lds = lds;
   e->h_syslist.assign(slots, slots + nslots);
   if ((int64_t)e->h_work.size() + 2 > e->work_cap) {
@@ -591,6 +648,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->p.nsys = e->p.nwork + 16;
     e->p.ncellwork = e->p.nwork + 32;
     e->p.nitems = e->p.nwork + 48;
+    e->p.nitems_far = e->p.nwork + 56;
     invalidate_graphs(e);
   }
   if ((int64_t)e->h_items.size() + 1 > e->items_cap) {
@@ -607,12 +665,14 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   if (ns > 0) HIPCK(hipMemcpy(e->p.syslist, e->h_syslist.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
   const int32_t ni = (int32_t)e->h_items.size();
   if (ni > 0) HIPCK(hipMemcpy(e->p.items, e->h_items.data(), sizeof(int4) * (size_t)ni, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.nitems_far, &n_far_items, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.nitems, &ni, 4, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.nitems, &n_lds_items, 4, hipMemcpyHostToDevice));
   e->cur_nwork = nw;
   e->cur_nsys = ns;
-  e->cur_nitems = ni;
+  e->cur_nitems = n_lds_items;
+  e->cur_nitems_far = n_far_items;
   return SOP_OK;
 }
 
@@ -640,7 +700,7 @@ sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslo
 static std::string graph_key(const sop_engine *e, int G, int n_nl, int n_pl) {
   std::ostringstream os;
   os << G << '|' << n_nl << '|' << n_pl << '|' << e->opts.lanes_per_bead << '|' << grid_tiles(e) << '|'
-     << grid_sys(e) << '|' << grid_items(e) << '|' << e->cur_lds;
+     << grid_sys(e) << '|' << grid_items(e) << '|' << grid_items_far(e) << '|' << e->cur_lds;
   return os.str();
 }
 
@@ -659,7 +719,7 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
     HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < G; ++k) {
       if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, gt, gs, k, 0, e->stream);
-      (void)launch_step(e->p, MODE_STEP, lpb, grid_items(e), k, e->cur_lds, e->stream);
+      (void)launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), k, e->cur_lds, e->stream);
     }
     (void)launch_advance(e->p, G, e->stream);
     HIPCK(hipStreamEndCapture(e->stream, &graph));
@@ -692,7 +752,7 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
       any_pl |= ((st[q] + k) % npl[q]) == 0;
     }
     if (any_nl || any_pl) HIPCK(launch_lists(e->p, gt, gs, (int)k, 0, e->stream));
-    HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), (int)k, e->cur_lds, e->stream));
+    HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), (int)k, e->cur_lds, e->stream));
   }
   HIPCK(launch_advance(e->p, (int)nsteps, e->stream));
   return SOP_OK;
@@ -719,7 +779,7 @@ void sop_engine_opts_default(sop_engine_opts *o) {
   o->lanes_per_bead = 16;
   o->grid_blocks = 0;
   o->use_graphs = 1;
-  o->lds_beads_max = 1024;
+  o->lds_beads_max = 5056;  // two 1024-thread step blocks per compute unit at 80,896 B each
   o->pool_bytes = 0;
 }
 
@@ -977,8 +1037,8 @@ sop_status sop_force_eval(sop_engine *e, int32_t sys, int32_t with_noise) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, grid_items(e), 0,
-                    e->cur_lds, e->stream));
+  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, grid_items(e),
+                    grid_items_far(e), 0, e->cur_lds, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -1153,7 +1213,8 @@ sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz) {
 static sop_status noisy_forces(sop_engine *e, int32_t sys, int kstep, std::vector<float4> &f) {
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_step(e->p, MODE_FORCE_NOISE, e->opts.lanes_per_bead, grid_items(e), kstep, e->cur_lds, e->stream));
+  HIPCK(launch_step(e->p, MODE_FORCE_NOISE, e->opts.lanes_per_bead, grid_items(e), grid_items_far(e), kstep, e->cur_lds,
+                    e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   st = engine_check_errors(e, &sys, 1);
   if (st) return st;
@@ -1301,7 +1362,7 @@ sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys,
   HIPCK(hipEventCreate(&b));
   const int lpb = e->opts.lanes_per_bead;
   HIPCK(hipEventRecord(a, e->stream));
-  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), r, e->cur_lds, e->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), r, e->cur_lds, e->stream));
   HIPCK(hipEventRecord(b, e->stream));
   HIPCK(hipEventSynchronize(b));
   float ms = 0.0f;
@@ -1319,19 +1380,19 @@ sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys,
 
 sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes, int32_t *grid) {
   if (!e) return SOP_ERR_INVALID;
-  const int32_t saved_l = e->opts.lanes_per_bead, saved_g = e->opts.grid_blocks;
+  const int32_t saved_l = e->opts.lanes_per_bead, saved_g = e->step_blocks;
   double best = 1e30;
   int32_t bl = saved_l, bg = saved_g;
-  for (int32_t l : {8, 16, 32}) {
-    for (int32_t gm : {4, 8, 16}) {
+  for (int32_t l : {4, 8, 16}) {
+    for (int32_t gm : {1, 2, 3, 4}) {
       e->opts.lanes_per_bead = l;
-      e->opts.grid_blocks = gm * e->num_cu;
+      e->step_blocks = gm * e->num_cu;
       double ms = 0.0;
       sop_status st = sop_time_step_kernel(e, sys, nsys, 3, &ms);  // warm
       if (st == SOP_OK) st = sop_time_step_kernel(e, sys, nsys, 10, &ms);
       if (st) {
         e->opts.lanes_per_bead = saved_l;
-        e->opts.grid_blocks = saved_g;
+        e->step_blocks = saved_g;
         return st;
       }
       if (ms < best) {
@@ -1342,7 +1403,7 @@ sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t
     }
   }
   e->opts.lanes_per_bead = bl;
-  e->opts.grid_blocks = bg;
+  e->step_blocks = bg;
   invalidate_graphs(e);
   if (lanes) *lanes = bl;
   if (grid) *grid = bg;

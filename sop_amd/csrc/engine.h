@@ -19,7 +19,8 @@
 namespace sopk {
 
 hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int flags, hipStream_t st);
-hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int kstep, size_t lds, hipStream_t st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int grid_far, int kstep, size_t lds,
+                       hipStream_t st);
 hipError_t step_lds_limit(size_t bytes);
 hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st);
 hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st);
@@ -84,9 +85,10 @@ struct sop_engine {
   std::vector<int32_t> h_work, h_syslist;
   std::vector<int4> h_items;
   int64_t work_cap = 0, syslist_cap = 0, items_cap = 0;
-  int32_t cur_nwork = 0, cur_nsys = 0, cur_nitems = 0;
+  int32_t cur_nwork = 0, cur_nsys = 0, cur_nitems = 0, cur_nitems_far = 0;
   size_t cur_lds = 0;     // dynamic LDS of the step kernel for the current batch
   size_t lds_limit = 0;   // largest dynamic LDS the step kernel may use
+  int32_t step_blocks = 0; // step-kernel grid (1024-thread blocks); 0 = 2 per compute unit
   bool fixed_grids = false; // serving: launch shapes independent of the batch, so
                             // one captured graph serves every batch composition
 
@@ -113,5 +115,6 @@ sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out);
 int grid_tiles(const sop_engine *e);
 int grid_sys(const sop_engine *e);
 int grid_items(const sop_engine *e);
+int grid_items_far(const sop_engine *e);
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d);
 } // namespace sopk

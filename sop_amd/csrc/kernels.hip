@@ -911,24 +911,25 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
 
 // ----------------------------------------------- fused force + integrate
 
-// Partner coordinates come from the system's LDS copy (LDS = true) or from
-// global memory.
+// Partner coordinates come from the LDS copy of the item's arena range
+// (LDS = true, base = first staged bead) or from global memory.
 template <bool LDS>
-__device__ __forceinline__ float4 partner(const float4 *__restrict__ pin, const float4 *s_pos, int bo, int j) {
-  return LDS ? s_pos[j - bo] : pin[j];
+__device__ __forceinline__ float4 partner(const float4 *__restrict__ pin, const float4 *s_pos, int base, int j) {
+  return LDS ? s_pos[j - base] : pin[j];
 }
 
-// One tile (64 beads) of a system: LPB lanes gather one bead's partners and
-// reduce; a 256-thread block covers the tile in LPB/4 rounds, stages the 64
-// forces in LDS, and wave 0 draws the random force and moves the 64 beads
-// (one bead per lane).
-template <int LPB, int MODE, bool LDS, bool IDX16>
-__device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int s, int t, int kstep,
+// One wave, one tile of 64 beads.  LPB lanes per bead gather its partners and
+// reduce with DPP; GPW = 64 / LPB beads per round, LPB rounds.  After each
+// round, lane m picks up the force of bead m of the tile from the group that
+// computed it, so the random force and the move run on all 64 lanes, one bead
+// per lane, with no block barrier.  ONE: every bead of the system has the
+// same type, so the force-field constants are wave-uniform (scalar registers).
+template <int LPB, int MODE, bool LDS, bool IDX16, bool ONE>
+__device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int s, int t, int kstep,
                                           const float4 *__restrict__ pin, float4 *__restrict__ pout,
-                                          float4 *sF, const float4 *s_pos) {
-  constexpr int GPW = 64 / LPB;            // bead groups per wave
-  constexpr int ROUNDS = TILE / (4 * GPW); // rounds to cover the tile
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+                                          const float4 *s_pos, int lds0) {
+  constexpr int GPW = 64 / LPB;
+  const int lane = threadIdx.x & 63;
   const int g = lane / LPB, l = lane % LPB;
   const int bead_off = S.bead_off, n = S.n;
   const int b0 = t * TILE;
@@ -936,20 +937,21 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
   const float boxl = S.boxl, inv_boxl = S.inv_boxl;
   const bool wrapf = S.wrap_free != 0 || boxl <= 0.0f;
   const int64_t step = S.step + kstep;
-  for (int r = 0; r < ROUNDS; ++r) {
-    const int lb = r * (4 * GPW) + wave * GPW + g;
-    const int i = b0 + lb;
+  const int q1 = S.type_mask == 2 ? 3 : 0;  // type-pair index of a one-type system
+  float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;  // force on bead b0 + lane
+  for (int r = 0; r < LPB; ++r) {
+    const int i = b0 + r * GPW + g;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (i < bend) {
-      const float4 xi = partner<LDS>(pin, s_pos, bead_off, i);
+      const float4 xi = partner<LDS>(pin, s_pos, lds0, i);
       const int wi = __float_as_int(xi.w);
-      const int row = ((wi & 3) - 1) * 2;
+      const int row = ONE ? q1 : ((wi & 3) - 1) * 2;
       // chain partners i-2, i-1, i+1, i+2: FENE bonds and i,i+2 soft sphere
       if (l < 4) {
         const int off = l < 2 ? l - 2 : l - 1;
         const int j = i + off;
         if (j >= bead_off && j < bead_off + n) {
-          const float4 xj = partner<LDS>(pin, s_pos, bead_off, j);
+          const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
           if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
             float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
             if (!wrapf) {
@@ -980,11 +982,11 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
       // native LJ over the native pair list
       {
         const int ns = p.nat_start[i], nc = p.platt_cnt[i];
-        const float eA = S.eps_att[row], eB = S.eps_att[row + 1];
+        const float eA = S.eps_att[row], eB = ONE ? eA : S.eps_att[row + 1];
         for (int k = l; k < nc; k += LPB) {
           const int j = p.platt_j[ns + k];
           const float r0 = p.platt_r0[ns + k];
-          const float4 xj = partner<LDS>(pin, s_pos, bead_off, j);
+          const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           if (!wrapf) {
             dx = mi_fast(dx, boxl, inv_boxl);
@@ -992,7 +994,7 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
             dz = mi_fast(dz, boxl, inv_boxl);
           }
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const float eps = w_type(xj.w) == 2 ? eB : eA;
+          const float eps = ONE ? eA : (w_type(xj.w) == 2 ? eB : eA);
           const float r02 = r0 * r0;
           const float ir2 = rcpf(d2);
           const float q2 = r02 * ir2;
@@ -1004,15 +1006,14 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
           fz = fmaf(c, dz, fz);
         }
       }
-      // non-native repulsion over the non-native pair list; four entries per
-      // lane per pass keep four independent gathers in flight
+      // non-native repulsion over the non-native pair list
       {
         const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
         const int nr = p.plrep_cnt[i];
-        const float s2A = S.sig2_rep[row], s2B = S.sig2_rep[row + 1];
-        const float m12A = S.rep_m12[row], m12B = S.rep_m12[row + 1];
-        const float m6A = S.rep_m6[row], m6B = S.rep_m6[row + 1];
-        const float gA = S.rep_guard[row], gB = S.rep_guard[row + 1];
+        const float s2A = S.sig2_rep[row], s2B = ONE ? s2A : S.sig2_rep[row + 1];
+        const float m12A = S.rep_m12[row], m12B = ONE ? m12A : S.rep_m12[row + 1];
+        const float m6A = S.rep_m6[row], m6B = ONE ? m6A : S.rep_m6[row + 1];
+        const float gA = S.rep_guard[row], gB = ONE ? gA : S.rep_guard[row + 1];
         const int32_t *__restrict__ lst = p.plrep_j + pb;
         const uint16_t *__restrict__ lst16 = (const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base);
         auto ent = [&](int k) -> int { return IDX16 ? bead_off + (int)lst16[k] : lst[k]; };
@@ -1024,7 +1025,7 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
             dz = mi_fast(dz, boxl, inv_boxl);
           }
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const bool tb = w_type(xj.w) == 2;
+          const bool tb = !ONE && w_type(xj.w) == 2;
           const float s2 = tb ? s2B : s2A;
           const float ir2 = rcpf(d2);
           const float q2 = s2 * ir2;
@@ -1035,109 +1036,136 @@ __device__ __forceinline__ void step_tile(const DevPtrs &p, const SysDev &S, int
           fy = fmaf(c, dy, fy);
           fz = fmaf(c, dz, fz);
         };
+        // U entries per lane per pass keep U gathers in flight (fewer from
+        // LDS, whose latency is short, to save registers)
+        constexpr int U = LDS ? 2 : 4;
         int k = l;
-        for (; k + 3 * LPB < nr; k += 4 * LPB) {
-          const int j0 = ent(k), j1 = ent(k + LPB), j2 = ent(k + 2 * LPB), j3 = ent(k + 3 * LPB);
-          const float4 x0 = partner<LDS>(pin, s_pos, bead_off, j0), x1 = partner<LDS>(pin, s_pos, bead_off, j1);
-          const float4 x2 = partner<LDS>(pin, s_pos, bead_off, j2), x3 = partner<LDS>(pin, s_pos, bead_off, j3);
-          rep(x0);
-          rep(x1);
-          rep(x2);
-          rep(x3);
+        for (; k + (U - 1) * LPB < nr; k += U * LPB) {
+          int jj[U];
+          float4 xx[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) jj[u] = ent(k + u * LPB);
+#pragma unroll
+          for (int u = 0; u < U; ++u) xx[u] = partner<LDS>(pin, s_pos, lds0, jj[u]);
+#pragma unroll
+          for (int u = 0; u < U; ++u) rep(xx[u]);
         }
-        for (; k < nr; k += LPB) rep(partner<LDS>(pin, s_pos, bead_off, ent(k)));
+        for (; k < nr; k += LPB) rep(partner<LDS>(pin, s_pos, lds0, ent(k)));
       }
     }
     fx = group_sum<LPB>(fx);
     fy = group_sum<LPB>(fy);
     fz = group_sum<LPB>(fz);
-    if (l == 0 && i < bend) sF[lb] = make_float4(fx, fy, fz, 0.0f);
-  }
-  __syncthreads();
-  if (wave == 0) {
-    const int i = b0 + lane;
-    if (i < bend) {
-      const float4 f = sF[lane];
-      if (MODE == MODE_FORCE) {
-        p.fout[i] = f;
-      } else {
-        float n0, n1, n2;
-        normals3(S.seed_lo, S.seed_hi, (uint32_t)(i - bead_off), step, n0, n1, n2);
-        const float var = S.var;
-        const float ftx = f.x + n0 * var, fty = f.y + n1 * var, ftz = f.z + n2 * var;
-        if (MODE == MODE_FORCE_NOISE) {
-          p.fout[i] = make_float4(ftx, fty, ftz, 0.0f);
-        } else {
-          float4 x = partner<LDS>(pin, s_pos, bead_off, i);
-          if (S.dyn == 0) {
-            // overdamped (sop.cu:487-520): x += (h/zeta) F
-            const float a5 = S.a5;
-            x.x = x.x + a5 * ftx;
-            x.y = x.y + a5 * fty;
-            x.z = x.z + a5 * ftz;
-          } else {
-            // underdamped (sop.cu:421-484): the reference's velocity update
-            // after its force evaluation, v = a3 incr + a4 F, happens here at
-            // the start of the next step, then incr = a1 v + a2 F, x += incr
-            const float4 st = p.vel[i];
-            float vx = st.x, vy = st.y, vz = st.z;
-            if (__float_as_int(st.w) == 1) {
-              vx = S.a3 * st.x + S.a4 * ftx;
-              vy = S.a3 * st.y + S.a4 * fty;
-              vz = S.a3 * st.z + S.a4 * ftz;
-            }
-            const float ix = S.a1 * vx + S.a2 * ftx, iy = S.a1 * vy + S.a2 * fty, iz = S.a1 * vz + S.a2 * ftz;
-            x.x = x.x + ix;
-            x.y = x.y + iy;
-            x.z = x.z + iz;
-            p.vel[i] = S.vupd ? make_float4(ix, iy, iz, __int_as_float(1)) : make_float4(vx, vy, vz, 0.0f);
-          }
-          pout[i] = x;
-          if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
-        }
-      }
+    const int src = (lane % GPW) * LPB;
+    const float vx = __shfl(fx, src), vy = __shfl(fy, src), vz = __shfl(fz, src);
+    if (lane / GPW == r) {
+      mfx = vx;
+      mfy = vy;
+      mfz = vz;
     }
   }
-  __syncthreads();
+  const int i = b0 + lane;
+  if (i >= bend) return;
+  if (MODE == MODE_FORCE) {
+    p.fout[i] = make_float4(mfx, mfy, mfz, 0.0f);
+    return;
+  }
+  float n0, n1, n2;
+  normals3(S.seed_lo, S.seed_hi, (uint32_t)(i - bead_off), step, n0, n1, n2);
+  const float var = S.var;
+  const float ftx = mfx + n0 * var, fty = mfy + n1 * var, ftz = mfz + n2 * var;
+  if (MODE == MODE_FORCE_NOISE) {
+    p.fout[i] = make_float4(ftx, fty, ftz, 0.0f);
+    return;
+  }
+  float4 x = partner<LDS>(pin, s_pos, lds0, i);
+  if (S.dyn == 0) {
+    // overdamped (sop.cu:487-520): x += (h/zeta) F
+    const float a5 = S.a5;
+    x.x = x.x + a5 * ftx;
+    x.y = x.y + a5 * fty;
+    x.z = x.z + a5 * ftz;
+  } else {
+    // underdamped (sop.cu:421-484): the reference's velocity update after its
+    // force evaluation, v = a3 incr + a4 F, happens here at the start of the
+    // next step, then incr = a1 v + a2 F, x += incr
+    const float4 st = p.vel[i];
+    float vx2 = st.x, vy2 = st.y, vz2 = st.z;
+    if (__float_as_int(st.w) == 1) {
+      vx2 = S.a3 * st.x + S.a4 * ftx;
+      vy2 = S.a3 * st.y + S.a4 * fty;
+      vz2 = S.a3 * st.z + S.a4 * ftz;
+    }
+    const float ix = S.a1 * vx2 + S.a2 * ftx, iy = S.a1 * vy2 + S.a2 * fty, iz = S.a1 * vz2 + S.a2 * ftz;
+    x.x = x.x + ix;
+    x.y = x.y + iy;
+    x.z = x.z + iz;
+    p.vel[i] = S.vupd ? make_float4(ix, iy, iz, __int_as_float(1)) : make_float4(vx2, vy2, vz2, 0.0f);
+  }
+  pout[i] = x;
+  if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
 }
 
-// Fused step over work items {system, first tile, tiles, flags}.  Items with
-// ITEM_LDS first copy their whole system's coordinates into dynamic LDS, so
-// every partner gather is an LDS read; the result is identical either way.
-template <int LPB, int MODE>
-__global__ __launch_bounds__(256) void k_step(DevPtrs p, int kstep) {
-  __shared__ float4 sF[TILE];
-  extern __shared__ float4 s_pos[];
+// Fused step over work items {first tile, tiles (<= STEP_WAVES), first staged
+// bead or -1, staged beads}: wave w of the block takes tile first + w.  The
+// LDS kernel (KLDS) takes the items whose systems fit in LDS and first copies
+// the staged arena beads (whole systems, each from its own coordinate buffer)
+// into dynamic LDS, so every partner gather is an LDS read; the other kernel
+// takes the items that gather from global memory (items[nitems .. nitems +
+// nitems_far)).  The result is identical either way.  Both are launched each
+// step, so each has its own register budget.
+template <int LPB, int MODE, bool KLDS>
+__device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *s_pos) {
   const int tid = threadIdx.x;
-  const int nitems = *p.nitems;
-  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: system data in scalar registers
+  const int w0 = KLDS ? 0 : *p.nitems;
+  const int w1 = KLDS ? *p.nitems : w0 + *p.nitems_far;
+  for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
-    const int s = it.x;
+    if (KLDS) {
+      __syncthreads();  // the previous item's reads are done
+      for (int b = tid; b < it.w; b += STEP_THREADS) {
+        const int a = it.z + b;  // 64 consecutive beads per wave: one tile, one system
+        const SysDev &Sb = p.sys[p.tile_sys[a / TILE]];
+        s_pos[b] = buf_in(p, Sb.parity + kstep)[a];
+      }
+      __syncthreads();
+    }
+    if (wave >= it.y) continue;
+    const int t = it.x + wave;
+    const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
     const int par = S.parity + kstep;
     const float4 *__restrict__ pin = buf_in(p, par);
     float4 *__restrict__ pout = buf_out(p, par);
     if (MODE == MODE_STEP && S.step + kstep >= S.step_end) {
       // finished inside this block of steps: carry the coordinates over
-      const int a = it.y * TILE, b = min((it.y + it.z) * TILE, S.bead_off + S.n);
-      for (int i = a + tid; i < b; i += blockDim.x) pout[i] = pin[i];
+      const int i = t * TILE + (tid & 63);
+      if (i < S.bead_off + S.n) pout[i] = pin[i];
       continue;
     }
-    if (it.w & ITEM_LDS) {
-      for (int b = tid; b < S.n; b += blockDim.x) s_pos[b] = pin[S.bead_off + b];
-      __syncthreads();
-      if (S.idx16)
-        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, true, true>(p, S, s, t, kstep, pin, pout, sF, s_pos);
-      else
-        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, true, false>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+    const int lds0 = KLDS ? it.z : 0;
+    if (S.type_mask != 3) {
+      if (S.idx16) wave_tile<LPB, MODE, KLDS, true, true>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
+      else wave_tile<LPB, MODE, KLDS, false, true>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
     } else {
-      if (S.idx16)
-        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, false, true>(p, S, s, t, kstep, pin, pout, sF, s_pos);
-      else
-        for (int t = it.y; t < it.y + it.z; ++t) step_tile<LPB, MODE, false, false>(p, S, s, t, kstep, pin, pout, sF, s_pos);
+      if (S.idx16) wave_tile<LPB, MODE, KLDS, true, false>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
+      else wave_tile<LPB, MODE, KLDS, false, false>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
     }
   }
+}
+
+// 8 waves per SIMD: two blocks per compute unit
+template <int LPB, int MODE>
+__global__ __launch_bounds__(STEP_THREADS, 8) void k_step(DevPtrs p, int kstep) {
+  extern __shared__ float4 s_pos[];
+  step_items<LPB, MODE, true>(p, kstep, s_pos);
+}
+
+template <int LPB, int MODE>
+__global__ __launch_bounds__(STEP_THREADS) void k_step_far(DevPtrs p, int kstep) {
+  step_items<LPB, MODE, false>(p, kstep, nullptr);
 }
 
 // ------------------------------------------------ energies / observables
@@ -1396,20 +1424,28 @@ hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kste
 }
 
 template <int MODE>
-static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid, int kstep, size_t lds, hipStream_t st) {
+static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid, int grid_far, int kstep, size_t lds,
+                                   hipStream_t st) {
   switch (lpb) {
-    case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
-    case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
-    case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
-    default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(256), lds, st, p, kstep); break;
+    case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
+    case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
+    case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
+    default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
+  }
+  switch (lpb) {
+    case 4: hipLaunchKernelGGL((k_step_far<4, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
+    case 8: hipLaunchKernelGGL((k_step_far<8, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
+    case 32: hipLaunchKernelGGL((k_step_far<32, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
+    default: hipLaunchKernelGGL((k_step_far<16, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
   }
   return SOPK_LAUNCH_CHECK();
 }
 
-hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int kstep, size_t lds, hipStream_t st) {
-  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, grid, kstep, lds, st);
-  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, grid, kstep, lds, st);
-  return launch_step_mode<MODE_STEP>(p, lpb, grid, kstep, lds, st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int grid_far, int kstep, size_t lds,
+                       hipStream_t st) {
+  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, grid, grid_far, kstep, lds, st);
+  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, grid, grid_far, kstep, lds, st);
+  return launch_step_mode<MODE_STEP>(p, lpb, grid, grid_far, kstep, lds, st);
 }
 
 // Allow the step kernels dynamic LDS up to `bytes`.

@@ -118,12 +118,15 @@ struct DevPtrs {
   int32_t *cellwork;    // per batch position: first home cell of the direct pair-list build
   int32_t *ncellwork;   // total home cells of the direct pair-list build
   int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
-  int4 *items;          // step-kernel work: {system slot, first tile, tiles, flags}
-  int32_t *nitems;
+  int4 *items;          // step-kernel work: {first tile, tiles, first staged bead or -1, staged beads}
+  int32_t *nitems;       // items that stage their systems in LDS: items[0 .. *nitems)
+  int32_t *nitems_far;   // items that gather from global memory: items[*nitems .. + *nitems_far)
 };
 
-constexpr int ITEM_LDS = 1;       // stage the system's coordinates in LDS
-constexpr int ITEM_TILES_MAX = 8; // tiles per step-kernel item
+// step kernel: blocks of STEP_WAVES waves, one 64-bead tile per wave; an item
+// is {first tile, tiles <= STEP_WAVES, first staged bead or -1, staged beads}
+constexpr int STEP_WAVES = 16;
+constexpr int STEP_THREADS = 64 * STEP_WAVES;
 
 constexpr int EPART_N = 6;   // bond, angle, native, rep, virial, contacts
 constexpr int SYSOUT_N = 16; // e[0..5], contacts, rgsq, sum x,y,z, n
