@@ -326,7 +326,7 @@ __global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep) {
 
 // Per system: exclusive scan of the histogram (DPP wave scan + wave totals).
 __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
-  __shared__ int wsum[4];
+  __shared__ int wsum[4], wused[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
@@ -334,12 +334,14 @@ __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
     SysDev &S = p.sys[s];
     if (S.build == BUILD_NONE) continue;
     const int nc = S.ncells, cb = S.cell_base;
-    int carry = 0;
+    int carry = 0, used = 0;
     for (int c0 = 0; c0 < nc; c0 += 256) {
       const int c = c0 + tid;
       const int v = c < nc ? p.cell_cnt[cb + c] : 0;
       const int incl = wave_incl_scan(v);
       if (lane == 63) wsum[wave] = incl;
+      const int wu = __popcll(__ballot(v > 0));
+      if (lane == 0) wused[wave] = wu;
       __syncthreads();
       int off = 0, tot = 0;
 #pragma unroll
@@ -352,9 +354,11 @@ __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
         p.cell_cursor[cb + c] = 0;
       }
       carry += tot;
+      used += wused[0] + wused[1] + wused[2] + wused[3];
       __syncthreads();
     }
     if (tid == 0) {
+      S.ncells_used = used;
       p.cell_start[cb + nc] = carry;
       if (carry != S.n) atomicOr(&S.err, ERR_INTERNAL);
     }
@@ -399,6 +403,7 @@ __global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep) {
         const int pos = st + rank;
         if (pos < S.n) {
           p.cell_pos[i] = pos;
+          p.nat_sorted[S.bead_off + pos] = make_int2(p.nat_start[i], p.nat_cnt[i]);
           p.cell_sorted[S.bead_off + pos] = i;
           p.pos_sorted[S.bead_off + pos] = build_src(p, S, kstep)[i];
         }
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(256) void k_cellwork(DevPtrs p) {
     int v = 0;
     if (q < nsys) {
       const SysDev &S = p.sys[p.syslist[q]];
-      v = S.build == BUILD_DIRECT ? S.ncells : 0;
+      v = S.build == BUILD_DIRECT ? S.ncells_used : 0;
     }
     const int incl = wave_incl_scan(v);
     if (lane == 63) wsum[wave] = incl;
@@ -522,12 +527,77 @@ __global__ __launch_bounds__(256) void k_cellwork(DevPtrs p) {
 
 // Per system of the batch: list its home cells for the direct build.
 __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
+  __shared__ int wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const SysDev &S = p.sys[p.syslist[q]];
     if (S.build != BUILD_DIRECT) continue;
-    const int off = p.cellwork[q], nc = S.ncells;
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) p.cellitems[off + c] = (q << 20) | c;
+    const int off = p.cellwork[q], nc = S.ncells, cb = S.cell_base;
+    int base = 0;
+    for (int c0 = 0; c0 < nc; c0 += 256) {
+      const int c = c0 + tid;
+      const bool used = c < nc && p.cell_cnt[cb + c] > 0;
+      const uint64_t m = __ballot(used);
+      if (lane == 0) wsum[wave] = __popcll(m);
+      __syncthreads();
+      int o = base;
+      for (int w = 0; w < wave; ++w) o += wsum[w];
+      if (used) p.cellitems[off + o + lanes_below(m)] = (q << 20) | c;
+      base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      __syncthreads();
+    }
+  }
+}
+
+// Per bead of a system that builds its pair list from the cell grid: the
+// native pair list (natives within their pair cutoff), and for every native
+// entry the partner's cell and sorted position, which k_pl_cells turns into
+// candidate slots.  16 lanes per bead.
+__global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gq = lane >> 4, gl = lane & 15;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const int s = p.tile_sys[t];
+    const SysDev &S = p.sys[s];
+    if (S.build != BUILD_DIRECT) continue;
+    const float4 *x = buf_in(p, S.parity + kstep);
+    const float boxl = S.boxl;
+    const double plf = S.pl_factor;
+    const float pla = S.pl_abs;
+    const int bend = min(t * TILE + TILE, S.bead_off + S.n);
+    for (int q = 0; q < TILE / 16; ++q) {
+      const int i = t * TILE + q * 16 + wave * 4 + gq;
+      const bool ok = i < bend;
+      const int ns = ok ? p.nat_start[i] : 0, nn = ok ? p.nat_cnt[i] : 0;
+      const float4 xi = ok ? x[i] : make_float4(0, 0, 0, 0);
+      int cnt = 0;
+      const int nmax = max(max(__shfl(nn, 0), __shfl(nn, 16)), max(__shfl(nn, 32), __shfl(nn, 48)));
+      for (int k0 = 0; k0 < nmax; k0 += 16) {
+        const int k = k0 + gl;
+        bool pred = false;
+        int j = 0;
+        float r0 = 0.0f;
+        if (k < nn) {
+          j = p.nat_j[ns + k];
+          r0 = p.nat_r0[ns + k];
+          p.natslot[ns + k] = make_int2(p.cell_of[j], p.cell_pos[j]);
+          const float rc = cutoff_of(plf, pla, r0);
+          pred = dist2_exact(xi, x[j], boxl) < rc * rc;
+        }
+        const uint32_t bits = (uint32_t)(__ballot(pred) >> (16 * gq)) & 0xFFFFu;
+        if (pred) {
+          const int o = ns + cnt + __popc(bits & ((1u << gl) - 1u));
+          p.platt_j[o] = j;
+          p.platt_r0[o] = r0;
+        }
+        cnt += __popc(bits);
+      }
+      if (ok && gl == 0) p.platt_cnt[i] = cnt;
+    }
   }
 }
 
@@ -610,11 +680,12 @@ __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_c
 __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
   __shared__ float4 s_home[HOME_MAX];
   __shared__ int s_hj[HOME_MAX], s_hcnt[HOME_MAX], s_hnn[HOME_MAX], s_hns[HOME_MAX];
-  __shared__ uint32_t s_excl[HOME_MAX][EXCL_WORDS];  // per home bead: candidate slots of its natives
+  __shared__ uint32_t s_excl[HOME_MAX][EXCL_WORDS];  // per home bead: candidate slots it must skip
   __shared__ float4 s_cand[CAND_MAX];
   __shared__ int s_cj[CAND_MAX];
-  __shared__ int s_rst[9], s_roff[10];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int s_rst[9], s_roff[10], s_hrange[2];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int total = *p.ncellwork;
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
     const int item = p.cellitems[g];
@@ -629,9 +700,7 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
                 rc2_22 = S.rc2_pl_rep[3];
     const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
     const int cx = c % d0, cy = (c / d0) % d1, cz = c / (d0 * d1);
-    const int hs = max(p.cell_start[cb + c], 0), he = min(p.cell_start[cb + c + 1], n);
-    if (hs >= he) continue;
-    // row bounds of the 3x3 neighbor rows (x-range cx-1..cx+1)
+    // bounds of the home cell and of the 3x3 neighbor rows (x-range cx-1..cx+1)
     if (tid < 9) {
       const int zz = cz + tid / 3 - 1, yy = cy + tid % 3 - 1;
       int st = 0, len = 0;
@@ -642,6 +711,8 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
       }
       s_rst[tid] = st;
       s_roff[tid + 1] = len;
+    } else if (tid < 11) {
+      s_hrange[tid - 9] = min(max(p.cell_start[cb + c + (tid - 9)], 0), n);
     }
     __syncthreads();
     if (tid == 0) {
@@ -649,7 +720,12 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
       for (int r = 1; r <= 9; ++r) s_roff[r] += s_roff[r - 1];
     }
     __syncthreads();
+    const int hs = s_hrange[0], he = s_hrange[1];
     const int T = s_roff[9];
+    if (hs >= he) {
+      __syncthreads();  // everyone has read the bounds before the next cell rewrites them
+      continue;
+    }
     const bool exact = T > CAND_MAX;
     // stage the candidates [b0, b0 + nc); the fast form pads to 128 with
     // coordinates no pair can reach
@@ -674,33 +750,42 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
     for (int h0 = hs; h0 < he; h0 += HOME_MAX) {
       const int nh = min(HOME_MAX, he - h0);
       if (tid < nh) {
-        const int j = p.cell_sorted[bo + h0 + tid];
+        const int2 ni = p.nat_sorted[bo + h0 + tid];
         s_home[tid] = p.pos_sorted[bo + h0 + tid];
-        s_hj[tid] = j;
+        s_hj[tid] = p.cell_sorted[bo + h0 + tid];
         s_hcnt[tid] = 0;
-        s_hns[tid] = p.nat_start[j];
-        s_hnn[tid] = p.nat_cnt[j];
+        s_hns[tid] = ni.x;
+        s_hnn[tid] = ni.y;
 #pragma unroll
         for (int w = 0; w < EXCL_WORDS; ++w) s_excl[tid][w] = 0u;
       }
       __syncthreads();
       if (!exact) {
         // exclusion bitmaps: the slot of each native partner and of the bead
-        // itself and its chain neighbours i +- 1, 2, from its cell in this
-        // rebuild's grid (the row) and its sorted position (the offset)
+        // itself and its chain neighbours i +- 1, 2, from their cells in this
+        // rebuild's grid (the row) and sorted positions (the offset)
         for (int h = wave; h < nh; h += 4) {
           const int ji = s_hj[h], ns = s_hns[h], nn = s_hnn[h];
           const int chi = __float_as_int(s_home[h].w) >> 2;
           for (int k = lane; k < nn + 5; k += 64) {
-            const int jn = k < nn ? p.nat_j[ns + k] : ji + (k - nn - 2);
-            if (jn < bo || jn >= bo + n) continue;
-            const int cn = p.cell_of[jn], pos = p.cell_pos[jn];
+            int cn, pos, jn;
+            if (k < nn) {
+              const int2 sl = p.natslot[ns + k];
+              cn = sl.x;
+              pos = sl.y;
+              jn = -1;
+            } else {
+              jn = ji + (k - nn - 2);
+              if (jn < bo || jn >= bo + n) continue;
+              cn = p.cell_of[jn];
+              pos = p.cell_pos[jn];
+            }
             const int nx = cn % d0, ny = (cn / d0) % d1, nz = cn / (d0 * d1);
             if (abs(nx - cx) > 1 || abs(ny - cy) > 1 || abs(nz - cz) > 1) continue;
             const int r = (nz - cz + 1) * 3 + (ny - cy + 1);
             const int slot = s_roff[r] + (pos - s_rst[r]);
-            if (slot < s_roff[r] || slot >= s_roff[r + 1] || s_cj[slot] != jn) continue;
-            if (k >= nn && (__float_as_int(s_cand[slot].w) >> 2) != chi) continue;  // another chain
+            if (slot < s_roff[r] || slot >= s_roff[r + 1]) continue;
+            if (jn >= 0 && (s_cj[slot] != jn || (__float_as_int(s_cand[slot].w) >> 2) != chi)) continue;
             atomicOr(&s_excl[h][slot >> 5], 1u << (slot & 31));
           }
         }
@@ -743,39 +828,10 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         }
       }
       __syncthreads();
-      // counts, and the native pair lists of the home beads
       if (tid < nh) {
         const int j = s_hj[tid], cr = s_hcnt[tid];
         p.plrep_cnt[j] = min(cr, pl_cap);
         if (cr > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
-      }
-      const float4 *x = buf_in(p, S.parity + kstep);
-      const double plf = S.pl_factor;
-      const float pla = S.pl_abs;
-      for (int h = wave; h < nh; h += 4) {
-        const int i = s_hj[h], ns = s_hns[h], nn = s_hnn[h];
-        const float4 xi = s_home[h];
-        int cnt = 0;
-        for (int k0 = 0; k0 < nn; k0 += 64) {
-          const int k = k0 + lane;
-          bool pred = false;
-          int j = 0;
-          float r0 = 0.0f;
-          if (k < nn) {
-            j = p.nat_j[ns + k];
-            r0 = p.nat_r0[ns + k];
-            const float rc = cutoff_of(plf, pla, r0);
-            pred = dist2_exact(xi, x[j], boxl) < rc * rc;
-          }
-          const uint64_t mm = __ballot(pred);
-          if (pred) {
-            const int o = ns + cnt + lanes_below(mm);
-            p.platt_j[o] = j;
-            p.platt_r0[o] = r0;
-          }
-          cnt += __popcll(mm);
-        }
-        if (lane == 0) p.platt_cnt[i] = cnt;
       }
       __syncthreads();
     }
@@ -1417,6 +1473,7 @@ hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kste
   hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cellwork, dim3(1), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_pl_natives, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);

@@ -66,6 +66,7 @@ struct SysDev {
   float inv_edge[3];
   int32_t dims[3];
   int32_t ncells;
+  int32_t ncells_used; // cells holding at least one bead
   int32_t wrap_free; // 1: every pair separation is below boxl/2 (minimum image is the identity)
   // list state, device-written.  The neighbor list is defined by the snapshot
   // pos_ref taken at its last rebuild; it is materialized only when needed.
@@ -105,6 +106,8 @@ struct DevPtrs {
   int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
   int32_t *cell_pos;    // per bead: its position in the system's cell-sorted order
+  int2 *nat_sorted;     // per sorted position: {nat_start, nat_cnt} of the bead there
+  int2 *natslot;        // per native CSR entry: {cell, sorted position} of the partner
   float4 *vel;          // underdamped state per bead: w = 1: xyz = last increment;
                         // w = 0: xyz = velocity entering the next step
   float4 *fout;         // forces (force-eval modes)
