@@ -91,6 +91,7 @@ int grid_items(const sop_engine *e) {
   int g = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
   g = std::max(8, g / 8 * 8);
   if (e->fixed_grids) return g;
+  if (e->cur_nitems == 0) return 0;  // no launch
   const int need = (e->cur_nitems + 7) / 8 * 8;
   return std::max(8, std::min(g, need));
 }
@@ -99,6 +100,7 @@ int grid_items_far(const sop_engine *e) {
   int g = 2 * e->num_cu;
   g = std::max(8, g / 8 * 8);
   if (e->fixed_grids) return g;
+  if (e->cur_nitems_far == 0) return 0;  // no launch
   const int need = (e->cur_nitems_far + 7) / 8 * 8;
   return std::max(8, std::min(g, need));
 }

@@ -95,6 +95,12 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -291,6 +297,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
         ext_max = fmaxf(ext_max, ext);
       }
       S.ncells = nc;
+      S.sort_lds = S.n <= SORT_N_MAX && nc <= SORT_C_MAX;
       if (S.boxl > 0.0f) {
         const int ok = ext_max < 0.5f * S.boxl - S.wrap_margin;
         S.wrap_free = ok;
@@ -314,7 +321,7 @@ __global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build == BUILD_NONE) continue;
+    if (S.build == BUILD_NONE || S.sort_lds) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = cell_index(S, build_src(p, S, kstep)[i]);
@@ -332,7 +339,7 @@ __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
-    if (S.build == BUILD_NONE) continue;
+    if (S.build == BUILD_NONE || S.sort_lds) continue;
     const int nc = S.ncells, cb = S.cell_base;
     int carry = 0, used = 0;
     for (int c0 = 0; c0 < nc; c0 += 256) {
@@ -372,7 +379,7 @@ __global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build == BUILD_NONE) continue;
+    if (S.build == BUILD_NONE || S.sort_lds) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = p.cell_of[i];
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build == BUILD_NONE) continue;
+    if (S.build == BUILD_NONE || S.sort_lds) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = p.cell_of[i];
@@ -492,6 +499,88 @@ __device__ __forceinline__ int scan_cells(const DevPtrs &p, const SysDev &S, int
   return cnt;
 }
 
+// Per system (one 1024-thread block) whose sort fits LDS: the whole counting
+// sort of the global kernels above in LDS -- cell of each bead, histogram,
+// exclusive scan, member lists, and the rank of each bead among its cell's
+// members by index, so the in-cell order is the index order -- then the
+// cell-ordered arrays written out in one pass.
+constexpr int SORT_THREADS = 1024;
+
+__global__ __launch_bounds__(SORT_THREADS) void k_cell_sort_lds(DevPtrs p, int kstep) {
+  __shared__ int s_cell[SORT_N_MAX];
+  __shared__ int s_mem[SORT_N_MAX];
+  __shared__ int s_cnt[SORT_C_MAX];  // counts, then end offsets
+  __shared__ int s_wsum[SORT_THREADS / 64], s_wused[SORT_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsys = *p.nsys;
+  for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
+    const int s = p.syslist[q];
+    SysDev &S = p.sys[s];
+    if (S.build == BUILD_NONE || !S.sort_lds) continue;
+    const int n = S.n, nc = S.ncells, bo = S.bead_off, cb = S.cell_base;
+    const float4 *src = build_src(p, S, kstep);
+    for (int c = tid; c < nc; c += SORT_THREADS) s_cnt[c] = 0;
+    __syncthreads();
+    for (int b = tid; b < n; b += SORT_THREADS) {
+      const int c = cell_index(S, src[bo + b]);
+      s_cell[b] = c;
+      p.cell_of[bo + b] = c;
+      atomicAdd(&s_cnt[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: 4 consecutive cells per thread
+    constexpr int PER = SORT_C_MAX / SORT_THREADS;
+    int v[PER], sum = 0, used = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int c = tid * PER + u;
+      v[u] = c < nc ? s_cnt[c] : 0;
+      sum += v[u];
+      used += v[u] > 0;
+    }
+    const int incl = wave_incl_scan(sum);
+    const int wu = wave_sum_i(used);
+    if (lane == 63) s_wsum[wave] = incl;
+    if (lane == 0) s_wused[wave] = wu;
+    __syncthreads();
+    int off = 0, tot_used = 0;
+    for (int w = 0; w < SORT_THREADS / 64; ++w) {
+      off += w < wave ? s_wsum[w] : 0;
+      tot_used += s_wused[w];
+    }
+    int run = off + incl - sum;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int c = tid * PER + u;
+      if (c < nc) {
+        p.cell_start[cb + c] = run;
+        s_cnt[c] = run;  // cursor, ends at the cell's end offset
+      }
+      run += v[u];
+    }
+    if (tid == 0) {
+      p.cell_start[cb + nc] = n;
+      S.ncells_used = tot_used;
+    }
+    __syncthreads();
+    for (int b = tid; b < n; b += SORT_THREADS) s_mem[atomicAdd(&s_cnt[s_cell[b]], 1)] = b;
+    __syncthreads();
+    for (int b = tid; b < n; b += SORT_THREADS) {
+      const int c = s_cell[b];
+      const int st = c > 0 ? s_cnt[c - 1] : 0, en = s_cnt[c];
+      int r = 0;
+      for (int m = st; m < en; ++m) r += s_mem[m] < b;
+      const int pos = st + r;
+      const int i = bo + b;
+      p.cell_pos[i] = pos;
+      p.cell_sorted[bo + pos] = i;
+      p.pos_sorted[bo + pos] = src[i];
+      p.nat_sorted[bo + pos] = make_int2(p.nat_start[i], p.nat_cnt[i]);
+    }
+    __syncthreads();
+  }
+}
+
 // Per batch: exclusive scan of the home-cell counts of the systems that build
 // their pair list directly (one block).
 __global__ __launch_bounds__(256) void k_cellwork(DevPtrs p) {
@@ -537,7 +626,7 @@ __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
     int base = 0;
     for (int c0 = 0; c0 < nc; c0 += 256) {
       const int c = c0 + tid;
-      const bool used = c < nc && p.cell_cnt[cb + c] > 0;
+      const bool used = c < nc && p.cell_start[cb + c + 1] > p.cell_start[cb + c];
       const uint64_t m = __ballot(used);
       if (lane == 0) wsum[wave] = __popcll(m);
       __syncthreads();
@@ -1467,6 +1556,7 @@ __global__ void k_selftest(int *out, float *fout) {
 hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int flags, hipStream_t st) {
   hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(256), 0, st, p, kstep, flags);
   hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_cell_sort_lds, dim3(grid_sys), dim3(SORT_THREADS), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cell_scan, dim3(grid_sys), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid_tiles), dim3(64), 0, st, p);
@@ -1483,13 +1573,13 @@ hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kste
 template <int MODE>
 static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid, int grid_far, int kstep, size_t lds,
                                    hipStream_t st) {
-  switch (lpb) {
+  if (grid > 0) switch (lpb) {
     case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
     case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
     case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
     default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
   }
-  switch (lpb) {
+  if (grid_far > 0) switch (lpb) {
     case 4: hipLaunchKernelGGL((k_step_far<4, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
     case 8: hipLaunchKernelGGL((k_step_far<8, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
     case 32: hipLaunchKernelGGL((k_step_far<32, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;

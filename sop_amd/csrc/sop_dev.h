@@ -67,6 +67,7 @@ struct SysDev {
   int32_t dims[3];
   int32_t ncells;
   int32_t ncells_used; // cells holding at least one bead
+  int32_t sort_lds;    // this rebuild's cell sort runs in one block's LDS
   int32_t wrap_free; // 1: every pair separation is below boxl/2 (minimum image is the identity)
   // list state, device-written.  The neighbor list is defined by the snapshot
   // pos_ref taken at its last rebuild; it is materialized only when needed.
@@ -130,6 +131,10 @@ struct DevPtrs {
 // is {first tile, tiles <= STEP_WAVES, first staged bead or -1, staged beads}
 constexpr int STEP_WAVES = 16;
 constexpr int STEP_THREADS = 64 * STEP_WAVES;
+
+// systems whose cell sort fits one block's LDS
+constexpr int SORT_N_MAX = 5120;
+constexpr int SORT_C_MAX = 4096;
 
 constexpr int EPART_N = 6;   // bond, angle, native, rep, virial, contacts
 constexpr int SYSOUT_N = 16; // e[0..5], contacts, rgsq, sum x,y,z, n
