@@ -54,12 +54,12 @@ def algorithmic_bytes(eng, ids, systems, n_pl, n_nl):
     return step, full / beads
 
 
-def cpu_baseline(systems, steps=300, warm=20, max_threads=16):
+def cpu_baseline(systems, steps=1000, warm=20, max_threads=16, every=8):
     from concurrent.futures import ThreadPoolExecutor
 
     import sop_amd as S
     from oracle import oracle as O
-    sample = systems[::32]
+    sample = systems[::every]
     ports = [O.Port(s) for s in sample]
     threads = max(1, min(max_threads, os.cpu_count() or 1, len(ports)))
 
@@ -77,7 +77,7 @@ def cpu_baseline(systems, steps=300, warm=20, max_threads=16):
     for p in ports:
         p.close()
     return {"value": beads * steps / dt, "unit": "bead-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample)} replicas of the workload (every 32nd, {beads} beads), {steps} steps after "
+            "sample": f"{len(sample)} replicas of the workload (every {every}th, {beads} beads), {steps} steps after "
                       f"{warm} warm-up, FP32 port of the reference host method (O(N^2) master-list filter, "
                       f"half-list force loops, overdamped step), {threads} threads over replicas",
             "seconds": dt}

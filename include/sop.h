@@ -254,6 +254,21 @@ sop_status sop_serve_poll(sop_engine *e, int64_t job, sop_job_result *r, float *
 sop_status sop_serve_stop(sop_engine *e);
 /* Device bytes a job holds while resident (the admission budget's unit). */
 int64_t sop_serve_job_bytes(const sop_engine *e, const sop_system_desc *d);
+/* Counters of the serve loop since sop_serve_start (no reference counterpart):
+ * where the loop's time goes, to tell a device-bound server from a host-bound
+ * one. */
+typedef struct sop_serve_stats {
+  int64_t iterations;     /* admission + block + retirement rounds */
+  int64_t jobs_admitted, jobs_retired;
+  int64_t steps_run;      /* sum over iterations of the block length */
+  double gpu_block_s;     /* device time of the step blocks (HIP events) */
+  double host_admit_s;    /* uploads and first lists of admitted jobs */
+  double host_batch_s;    /* work tables of the running batch */
+  double host_block_s;    /* wall time around the step blocks */
+  double host_retire_s;   /* energies, copies back, frees */
+  double mean_running;    /* running jobs per iteration, averaged */
+} sop_serve_stats;
+sop_status sop_serve_stats_get(sop_engine *e, sop_serve_stats *out);
 double sop_clock_seconds(void);
 
 /* --- host-only utilities (no device needed) ------------------------------ */

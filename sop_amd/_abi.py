@@ -86,6 +86,13 @@ class JobResult(C.Structure):
     ]
 
 
+class ServeStats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("jobs_admitted", C.c_int64), ("jobs_retired", C.c_int64),
+                ("steps_run", C.c_int64), ("gpu_block_s", C.c_double), ("host_admit_s", C.c_double),
+                ("host_batch_s", C.c_double), ("host_block_s", C.c_double), ("host_retire_s", C.c_double),
+                ("mean_running", C.c_double)]
+
+
 class GenOpts(C.Structure):
     _fields_ = [("bond", C.c_float), ("min_dist", C.c_float), ("volume_per_bead", C.c_float),
                 ("relax_steps", C.c_int32)]
@@ -134,6 +141,7 @@ SIGNATURES = {
     "sop_serve_poll": (i32, [vp, i64, P(JobResult), P(f32)]),
     "sop_serve_stop": (i32, [vp]),
     "sop_serve_job_bytes": (i64, [vp, P(SystemDesc)]),
+    "sop_serve_stats_get": (i32, [vp, P(ServeStats)]),
     "sop_clock_seconds": (f64, []),
     "sop_gen_opts_default": (None, [P(GenOpts)]),
     "sop_generate_chain": (i32, [u64, i32, P(GenOpts), P(f32), P(f32)]),

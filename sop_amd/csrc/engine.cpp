@@ -269,8 +269,8 @@ int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
   return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + pl_cap) + 12 * cell_cap;
 }
 
-// Validate and upload one system; builds its initial neighbor and pair lists.
-sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *slot_out) {
+// Validate and upload one system, without its first lists (engine_init_lists).
+sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, int64_t step_end, int32_t *slot_out) {
   if (!d || d->n_beads < 1 || !d->xyz) return set_error(e, SOP_ERR_INVALID, "system: need n_beads >= 1 and xyz");
   const int32_t n = d->n_beads;
   const sop_params &P = d->params;
@@ -426,7 +426,7 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   S.dmax = dmax;
   S.parity = 0;
   S.step = d->step0;
-  S.step_end = d->step0;
+  S.step_end = step_end > d->step0 ? step_end : d->step0;
   S.n_nl = P.n_nl;
   S.n_pl = P.n_pl;
   float a5, var;
@@ -504,16 +504,34 @@ sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *s
   S.wrap_free = 0;
   HIPCK(hipMemcpy(p.sys + slot, &S, sizeof(SysDev), hipMemcpyHostToDevice));
   e->sys[slot] = std::move(h);
-  // initial lists (the reference builds both before its step loop)
-  st = engine_set_batch(e, &slot, 1);
+  *slot_out = slot;
+  return SOP_OK;
+}
+
+// First neighbor and pair lists of newly added systems, in one launch
+// sequence (the reference builds both before its step loop).  status[q]
+// receives each system's result; systems that fail are removed.
+sop_status engine_init_lists(sop_engine *e, const int32_t *slots, int32_t n, sop_status *status) {
+  if (n <= 0) return SOP_OK;
+  sop_status st = engine_set_batch(e, slots, n);
   if (st) return st;
   HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_NL | FORCE_PL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
-  st = engine_check_errors(e, &slot, 1);
-  if (st) {
-    engine_remove_system(e, slot);
-    return st;
+  for (int32_t q = 0; q < n; ++q) {
+    status[q] = engine_check_errors(e, &slots[q], 1);
+    if (status[q]) engine_remove_system(e, slots[q]);
   }
+  return SOP_OK;
+}
+
+sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *slot_out) {
+  int32_t slot = -1;
+  sop_status st = engine_add_system_nolists(e, d, 0, &slot);
+  if (st) return st;
+  sop_status one = SOP_OK;
+  st = engine_init_lists(e, &slot, 1, &one);
+  if (st) return st;
+  if (one) return one;
   *slot_out = slot;
   return SOP_OK;
 }

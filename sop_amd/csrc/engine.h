@@ -106,6 +106,8 @@ namespace sopk {
 sop_status set_error(sop_engine *e, sop_status st, const std::string &msg);
 sop_status hip_check(sop_engine *e, hipError_t h, const char *what);
 sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *slot);
+sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, int64_t step_end, int32_t *slot);
+sop_status engine_init_lists(sop_engine *e, const int32_t *slots, int32_t n, sop_status *status);
 sop_status engine_remove_system(sop_engine *e, int32_t slot);
 sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots);
 sop_status engine_run_block_graph(sop_engine *e, int32_t G);

@@ -47,6 +47,8 @@ struct sop_engine::Serve {
   int32_t max_jobs = 0, block = 0, n_nl = 0, n_pl = 0;
   bool schedule_set = false;
   std::string fatal;
+  sop_serve_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 namespace {
@@ -115,6 +117,7 @@ void serve_loop(sop_engine *e) {
   (void)hipSetDevice(e->device);
   for (;;) {
     // 1. admission
+    const double t0 = sop_clock_seconds();
     std::vector<sop_engine::Job *> admit;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
@@ -130,12 +133,16 @@ void serve_loop(sop_engine *e) {
         admit.push_back(j);
       }
     }
+    // upload every admitted job (each runs from its step0 to step0 + nsteps),
+    // then build all their first lists with one launch sequence
+    std::vector<sop_engine::Job *> added;
+    std::vector<int32_t> added_slots;
     for (sop_engine::Job *j : admit) {
       int32_t slot = -1;
-      sop_status st = engine_add_system(e, &j->desc, &slot);
-      std::lock_guard<std::mutex> lk(sv->mu);
-      j->t_admit = sop_clock_seconds();
+      const sop_status st = engine_add_system_nolists(e, &j->desc, j->desc.step0 + j->nsteps, &slot);
       if (st != SOP_OK) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        j->t_admit = sop_clock_seconds();
         j->status = st;
         j->done = 1;
         j->t_done = j->t_admit;
@@ -143,23 +150,67 @@ void serve_loop(sop_engine *e) {
         continue;
       }
       j->slot = slot;
-      // the job runs from its step0 to step0 + nsteps
-      const int64_t end = j->desc.step0 + j->nsteps;
-      if (hipMemcpy((char *)(e->p.sys + slot) + offsetof(SysDev, step_end), &end, 8, hipMemcpyHostToDevice) !=
-          hipSuccess) {
-        sv->fatal = "step_end upload failed";
+      added.push_back(j);
+      added_slots.push_back(slot);
+    }
+    if (!added.empty()) {
+      std::vector<sop_status> ok(added.size(), SOP_OK);
+      if (engine_init_lists(e, added_slots.data(), (int32_t)added.size(), ok.data()) != SOP_OK) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->fatal = e->err;
         return;
       }
-      sv->running.push_back(j);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      const double now = sop_clock_seconds();
+      for (size_t q = 0; q < added.size(); ++q) {
+        sop_engine::Job *j = added[q];
+        j->t_admit = now;
+        if (ok[q] != SOP_OK) {
+          j->status = ok[q];
+          j->done = 1;
+          j->t_done = now;
+          j->slot = -1;
+          sv->used -= j->bytes;
+          continue;
+        }
+        sv->running.push_back(j);
+      }
     }
-    if (sv->running.empty()) continue;
+    const double t1 = sop_clock_seconds();
+    if (sv->running.empty()) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->stats.host_admit_s += t1 - t0;
+      continue;
+    }
     // 2. one block of steps over the running batch
     std::vector<int32_t> slots;
     for (sop_engine::Job *j : sv->running) slots.push_back(j->slot);
     sop_status st = engine_set_batch(e, slots.data(), (int32_t)slots.size());
+    const double t2 = sop_clock_seconds();
+    if (st == SOP_OK && sv->ev0) (void)hipEventRecord(sv->ev0, e->stream);
     if (st == SOP_OK) st = engine_run_block_graph(e, sv->block);
+    if (st == SOP_OK && sv->ev1) (void)hipEventRecord(sv->ev1, e->stream);
+    const double t3 = sop_clock_seconds();
     // 3. retire finished jobs
+    const size_t before = sv->running.size();
     if (st == SOP_OK) st = retire(e, sv);
+    const double t4 = sop_clock_seconds();
+    if (st == SOP_OK) {
+      float ms = 0.0f;
+      if (sv->ev0 && hipEventElapsedTime(&ms, sv->ev0, sv->ev1) != hipSuccess) ms = 0.0f;
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sop_serve_stats &c = sv->stats;
+      c.mean_running = (c.mean_running * (double)c.iterations + (double)before) / (double)(c.iterations + 1);
+      c.iterations += 1;
+      c.jobs_admitted += (int64_t)admit.size();
+      c.jobs_retired += (int64_t)(before - sv->running.size());
+      c.steps_run += sv->block;
+      c.gpu_block_s += 1e-3 * ms;
+      c.host_admit_s += t1 - t0;
+      c.host_batch_s += t2 - t1;
+      c.host_block_s += t3 - t2;
+      c.host_retire_s += t4 - t3;
+    }
     if (st != SOP_OK) {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->fatal = e->err;
@@ -187,6 +238,7 @@ sop_status sop_serve_start(sop_engine *e, const sop_serve_opts *o) {
   sv->block = (o && o->block_steps > 0) ? o->block_steps : 0;
   e->serve = sv;
   e->fixed_grids = true;
+  if (hipEventCreate(&sv->ev0) != hipSuccess || hipEventCreate(&sv->ev1) != hipSuccess) sv->ev0 = sv->ev1 = nullptr;
   sv->th = std::thread(serve_loop, e);
   return SOP_OK;
 }
@@ -269,6 +321,15 @@ int64_t sop_serve_job_bytes(const sop_engine *e, const sop_system_desc *d) {
   return job_bytes_estimate(e, d);
 }
 
+sop_status sop_serve_stats_get(sop_engine *e, sop_serve_stats *out) {
+  if (!e || !out) return SOP_ERR_INVALID;
+  sop_engine::Serve *sv = e->serve;
+  if (!sv) return SOP_ERR_STATE;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  *out = sv->stats;
+  return SOP_OK;
+}
+
 sop_status sop_serve_stop(sop_engine *e) {
   if (!e) return SOP_ERR_INVALID;
   sop_engine::Serve *sv = e->serve;
@@ -283,6 +344,8 @@ sop_status sop_serve_stop(sop_engine *e) {
   for (sop_engine::Job *j : sv->running)
     if (j->slot >= 0) engine_remove_system(e, j->slot);
   const bool ok = sv->fatal.empty();
+  if (sv->ev0) (void)hipEventDestroy(sv->ev0);
+  if (sv->ev1) (void)hipEventDestroy(sv->ev1);
   delete sv;
   e->serve = nullptr;
   e->fixed_grids = false;

@@ -56,6 +56,11 @@ class Server:
                 raise TimeoutError(f"job {job} not done after {timeout}s")
             time.sleep(0.001)
 
+    def stats(self) -> dict:
+        st = _abi.ServeStats()
+        self.eng._c(lib().sop_serve_stats_get(self.eng._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _abi.ServeStats._fields_}
+
     def stop(self):
         self.eng._c(lib().sop_serve_stop(self.eng._h))
 
@@ -105,6 +110,7 @@ def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates
         pending = keep
         if pending:
             time.sleep(0.01)
+    loop = srv.stats()
     srv.stop()
     eng.close()
     wall = t_last - clock0
@@ -115,7 +121,8 @@ def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates
            "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
            "bead_steps_per_s": done_beadsteps / wall if wall > 0 else None,
-           "offered_bead_steps_per_s": offered, "wall_s": wall}
+           "offered_bead_steps_per_s": offered, "wall_s": wall,
+           "loop": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in loop.items()}}
     log(f"served: {out['completed']}/{out['jobs']} done, p50 {out['p50_latency_s']:.3f}s "
         f"p99 {out['p99_latency_s']:.3f}s, {out['bead_steps_per_s']:.4g} bead-steps/s "
         f"(offered {offered:.4g})")

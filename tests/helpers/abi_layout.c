@@ -20,5 +20,7 @@ int main(void) {
   F(sop_job_result, steps); F(sop_job_result, t_submit); F(sop_job_result, energies);
   printf("sop_list_stats %zu\n", sizeof(sop_list_stats));
   printf("sop_gen_opts %zu\n", sizeof(sop_gen_opts));
+  printf("sop_serve_stats %zu\n", sizeof(sop_serve_stats));
+  F(sop_serve_stats, gpu_block_s); F(sop_serve_stats, mean_running);
   return 0;
 }
