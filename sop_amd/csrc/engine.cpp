@@ -124,7 +124,8 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.platt_cnt, 4},  {(void **)&p.nlrep_cnt, 4},   {(void **)&p.plrep_cnt, 4},
       {(void **)&p.cell_of, 4},    {(void **)&p.cell_tmp, 4},    {(void **)&p.cell_sorted, 4},
       {(void **)&p.cell_pos, 4},   {(void **)&p.nat_sorted, 8},
-      {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16}};
+      {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
+      {(void **)&p.tile_ord, 1}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},   {(void **)&p.nat_r0, 4},   {(void **)&p.nlatt_j, 4}, {(void **)&p.natslot, 8},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
@@ -409,6 +410,12 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   }
   HIPCK(hipMemcpy(p.nat_j + h.nat_off, natj.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(p.nat_r0 + h.nat_off, natr.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
+  {
+    // identity bead order per tile until the first pair list orders it
+    std::vector<uint8_t> ord((size_t)h.bead_len);
+    for (size_t k = 0; k < ord.size(); ++k) ord[k] = (uint8_t)(k % TILE);
+    HIPCK(hipMemcpy(p.tile_ord + b0, ord.data(), ord.size(), hipMemcpyHostToDevice));
+  }
   std::vector<int32_t> ts((size_t)(h.bead_len / TILE), slot);
   HIPCK(hipMemcpy(p.tile_sys + b0 / TILE, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice));
   // device descriptor

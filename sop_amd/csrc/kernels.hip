@@ -1054,6 +1054,30 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
   }
 }
 
+// Per tile of a system whose pair list was rebuilt: the order in which the
+// step kernel's lane groups take the tile's beads, by decreasing pair count
+// (non-native + native), ties by index, so the beads that one round of lane
+// groups computes together run loops of similar length.  The forces do not
+// depend on the order: each bead is summed by one group the same way.
+__global__ __launch_bounds__(256) void k_tile_order(DevPtrs p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x * 4 + wave; w < nwork; w += gridDim.x * 4) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const SysDev &S = p.sys[p.tile_sys[t]];
+    if (S.build != BUILD_DIRECT && S.build != BUILD_NL) continue;
+    const int b0 = t * TILE, i = b0 + lane;
+    const int key = i < S.bead_off + S.n ? p.plrep_cnt[i] + p.platt_cnt[i] : -1;
+    int rank = 0;
+    for (int j = 0; j < TILE; ++j) {
+      const int kj = __shfl(key, j);
+      rank += (kj > key) | ((kj == key) & (j < lane));
+    }
+    p.tile_ord[b0 + rank] = (uint8_t)lane;
+  }
+}
+
 // ----------------------------------------------- fused force + integrate
 
 // Partner coordinates come from the LDS copy of the item's arena range
@@ -1079,15 +1103,46 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   const int bead_off = S.bead_off, n = S.n;
   const int b0 = t * TILE;
   const int bend = min(b0 + TILE, bead_off + n);
-  const float boxl = S.boxl, inv_boxl = S.inv_boxl;
-  const bool wrapf = S.wrap_free != 0 || boxl <= 0.0f;
+  // No minimum image: a state whose extent reaches boxl/2 - margin is refused
+  // at each rebuild (ERR_PBC), so every separation here is below boxl/2.
   const int64_t step = S.step + kstep;
   const int q1 = S.type_mask == 2 ? 3 : 0;  // type-pair index of a one-type system
+  // slot q of the tile's order (k_tile_order) is computed in round q / GPW by
+  // group q % GPW; lane m's own bead b0 + m sits in slot qm
+  const int ordm = p.tile_ord[b0 + lane];
+  const int qm = __builtin_amdgcn_ds_permute(ordm << 2, lane);
+  const int src = (qm % GPW) * LPB, rm = qm / GPW;
+  // per-bead list heads of the whole tile in one coalesced load each; a
+  // round picks its beads' values up by shuffles
+  int t_ns = 0, t_nc = 0, t_nr = 0;
+  if (b0 + lane < bend) {
+    t_ns = p.nat_start[b0 + lane];
+    t_nc = p.platt_cnt[b0 + lane];
+    t_nr = p.plrep_cnt[b0 + lane];
+  }
+  // the non-native rows in 16-byte chunks of E entries
+  constexpr int E = IDX16 ? 8 : 4;
   float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;  // force on bead b0 + lane
   for (int r = 0; r < LPB; ++r) {
-    const int i = b0 + r * GPW + g;
+    const int li = __shfl(ordm, r * GPW + g);
+    const int i = b0 + li;
+    const int ns = __shfl(t_ns, li), nc = __shfl(t_nc, li), nr = __shfl(t_nr, li);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (i < bend) {
+      // first loads of the native and non-native loops, issued before any gather
+      const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
+      const uint4 *__restrict__ row4 =
+          IDX16 ? reinterpret_cast<const uint4 *>((const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base))
+                : reinterpret_cast<const uint4 *>(p.plrep_j + pb);
+      const int nch = (nr + E - 1) / E;
+      uint4 q = make_uint4(0u, 0u, 0u, 0u);
+      if (l < nch) q = row4[l];
+      int jn = 0;
+      float r0n = 0.0f;
+      if (l < nc) {
+        jn = p.platt_j[ns + l];
+        r0n = p.platt_r0[ns + l];
+      }
       const float4 xi = partner<LDS>(pin, s_pos, lds0, i);
       const int wi = __float_as_int(xi.w);
       const int row = ONE ? q1 : ((wi & 3) - 1) * 2;
@@ -1099,11 +1154,6 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
           if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
             float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            if (!wrapf) {
-              dx = mi_fast(dx, boxl, inv_boxl);
-              dy = mi_fast(dy, boxl, inv_boxl);
-              dz = mi_fast(dz, boxl, inv_boxl);
-            }
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float c;
             if (off == 1 || off == -1) {
@@ -1124,20 +1174,18 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           }
         }
       }
-      // native LJ over the native pair list
+      // native LJ over the native pair list, the next entry's loads ahead
       {
-        const int ns = p.nat_start[i], nc = p.platt_cnt[i];
         const float eA = S.eps_att[row], eB = ONE ? eA : S.eps_att[row + 1];
         for (int k = l; k < nc; k += LPB) {
-          const int j = p.platt_j[ns + k];
-          const float r0 = p.platt_r0[ns + k];
+          const int j = jn;
+          const float r0 = r0n;
+          if (k + LPB < nc) {
+            jn = p.platt_j[ns + k + LPB];
+            r0n = p.platt_r0[ns + k + LPB];
+          }
           const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-          if (!wrapf) {
-            dx = mi_fast(dx, boxl, inv_boxl);
-            dy = mi_fast(dy, boxl, inv_boxl);
-            dz = mi_fast(dz, boxl, inv_boxl);
-          }
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const float eps = ONE ? eA : (w_type(xj.w) == 2 ? eB : eA);
           const float r02 = r0 * r0;
@@ -1151,24 +1199,17 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           fz = fmaf(c, dz, fz);
         }
       }
-      // non-native repulsion over the non-native pair list
+      // non-native repulsion: lane l takes chunks l, l + LPB, ... of the row,
+      // one load per E pairs, the next chunk's load issued before the current
+      // chunk's gathers; entries past the count gather the bead itself and
+      // contribute nothing
       {
-        const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
-        const int nr = p.plrep_cnt[i];
         const float s2A = S.sig2_rep[row], s2B = ONE ? s2A : S.sig2_rep[row + 1];
         const float m12A = S.rep_m12[row], m12B = ONE ? m12A : S.rep_m12[row + 1];
         const float m6A = S.rep_m6[row], m6B = ONE ? m6A : S.rep_m6[row + 1];
         const float gA = S.rep_guard[row], gB = ONE ? gA : S.rep_guard[row + 1];
-        const int32_t *__restrict__ lst = p.plrep_j + pb;
-        const uint16_t *__restrict__ lst16 = (const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base);
-        auto ent = [&](int k) -> int { return IDX16 ? bead_off + (int)lst16[k] : lst[k]; };
-        auto rep = [&](float4 xj) {
+        auto rep = [&](float4 xj, bool ok) {
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-          if (!wrapf) {
-            dx = mi_fast(dx, boxl, inv_boxl);
-            dy = mi_fast(dy, boxl, inv_boxl);
-            dz = mi_fast(dz, boxl, inv_boxl);
-          }
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const bool tb = !ONE && w_type(xj.w) == 2;
           const float s2 = tb ? s2B : s2A;
@@ -1176,34 +1217,29 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           const float q2 = s2 * ir2;
           const float q6 = q2 * q2 * q2;
           float c = ir2 * q6 * fmaf(tb ? m12B : m12A, q6, tb ? m6B : m6A);
-          c = d2 > (tb ? gB : gA) ? c : 0.0f;
+          c = ok && d2 > (tb ? gB : gA) ? c : 0.0f;
           fx = fmaf(c, dx, fx);
           fy = fmaf(c, dy, fy);
           fz = fmaf(c, dz, fz);
         };
-        // U entries per lane per pass keep U gathers in flight (fewer from
-        // LDS, whose latency is short, to save registers)
-        constexpr int U = LDS ? 2 : 4;
-        int k = l;
-        for (; k + (U - 1) * LPB < nr; k += U * LPB) {
-          int jj[U];
-          float4 xx[U];
+        for (int c = l; c < nch; c += LPB) {
+          const uint4 cur = q;
+          if (c + LPB < nch) q = row4[c + LPB];
+          const uint32_t ww[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll
-          for (int u = 0; u < U; ++u) jj[u] = ent(k + u * LPB);
-#pragma unroll
-          for (int u = 0; u < U; ++u) xx[u] = partner<LDS>(pin, s_pos, lds0, jj[u]);
-#pragma unroll
-          for (int u = 0; u < U; ++u) rep(xx[u]);
+          for (int e = 0; e < E; ++e) {
+            const int j = IDX16 ? bead_off + (int)((ww[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu) : (int)ww[e];
+            const bool ok = c * E + e < nr;
+            rep(partner<LDS>(pin, s_pos, lds0, ok ? j : i), ok);
+          }
         }
-        for (; k < nr; k += LPB) rep(partner<LDS>(pin, s_pos, lds0, ent(k)));
       }
     }
     fx = group_sum<LPB>(fx);
     fy = group_sum<LPB>(fy);
     fz = group_sum<LPB>(fz);
-    const int src = (lane % GPW) * LPB;
     const float vx = __shfl(fx, src), vy = __shfl(fy, src), vz = __shfl(fz, src);
-    if (lane / GPW == r) {
+    if (rm == r) {
       mfx = vx;
       mfy = vy;
       mfz = vz;
@@ -1567,6 +1603,7 @@ hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kste
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
   return SOPK_LAUNCH_CHECK();
 }
 

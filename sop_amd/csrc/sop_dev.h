@@ -115,6 +115,7 @@ struct DevPtrs {
   double *epart;        // per bead: 6 energy partials
   double *sysout;       // per system slot: SYSOUT_N reduced values
   int32_t *tile_sys;    // system slot of each tile
+  uint8_t *tile_ord;    // per tile: TILE bead offsets in the order the step kernel's lane groups take them
   int32_t *work;        // tile ids to process (-1 = padding), XCD-interleaved
   int32_t *nwork;
   int32_t *syslist;     // system slots of the current batch
