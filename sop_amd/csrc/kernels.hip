@@ -20,6 +20,7 @@
 // one launch covers any batch of systems and a captured graph stays valid when
 // the batch changes.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "sop_dev.h"
 
@@ -699,6 +700,20 @@ constexpr int HOME_MAX = 64;   // home beads per pass
 constexpr int CAND_MAX = 1024; // candidates staged per pass (and bits of the exclusion bitmaps)
 constexpr int EXCL_WORDS = CAND_MAX / 32;
 
+// Pad a pair-list row of cnt entries with the bead's own index up to a
+// multiple of ROW_PAD, so the step kernel reads whole chunks without a count
+// test: a self entry has d2 = 0, which the repulsion guard turns into zero
+// force.  pl_cap is a multiple of 16, so the padding stays inside the row.
+constexpr int ROW_PAD = 4;
+__device__ __forceinline__ void pad_row(const DevPtrs &p, const SysDev &S, int i, int cnt, int k) {
+  const int e = cnt + k;
+  if (k < ((ROW_PAD - (cnt & (ROW_PAD - 1))) & (ROW_PAD - 1)) && e < S.pl_cap) {
+    const int64_t row = (int64_t)(i - S.bead_off) * S.pl_cap;
+    if (S.idx16) ((uint16_t *)(p.plrep_j + S.pl_base))[row + e] = (uint16_t)(i - S.bead_off);
+    else p.plrep_j[S.pl_base + row + e] = i;
+  }
+}
+
 // One home bead against the nc staged candidates [b0, b0 + nc), two per lane
 // per pass with no branch in the loop.  Fast form (EXACT = false): the staged
 // arrays are padded to a multiple of 128 with unreachable coordinates, and the
@@ -921,6 +936,7 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         const int j = s_hj[tid], cr = s_hcnt[tid];
         p.plrep_cnt[j] = min(cr, pl_cap);
         if (cr > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+        for (int k = 0; k < ROW_PAD - 1; ++k) pad_row(p, S, j, min(cr, pl_cap), k);
       }
       __syncthreads();
     }
@@ -1050,6 +1066,7 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
         p.plrep_cnt[i] = min(cr, S.pl_cap);
         if (cr > S.pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
       }
+      pad_row(p, S, i, min(cr, S.pl_cap), lane);
     }
   }
 }
@@ -1120,8 +1137,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     t_nc = p.platt_cnt[b0 + lane];
     t_nr = p.plrep_cnt[b0 + lane];
   }
-  // the non-native rows in 16-byte chunks of E entries
-  constexpr int E = IDX16 ? 8 : 4;
+  // the non-native rows in chunks of E = ROW_PAD entries (8 or 16 bytes)
+  constexpr int E = ROW_PAD;
+  using chunk_t = typename std::conditional<IDX16, uint2, uint4>::type;
   float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;  // force on bead b0 + lane
   for (int r = 0; r < LPB; ++r) {
     const int li = __shfl(ordm, r * GPW + g);
@@ -1131,12 +1149,12 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     if (i < bend) {
       // first loads of the native and non-native loops, issued before any gather
       const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
-      const uint4 *__restrict__ row4 =
-          IDX16 ? reinterpret_cast<const uint4 *>((const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base))
-                : reinterpret_cast<const uint4 *>(p.plrep_j + pb);
+      const chunk_t *__restrict__ rowc =
+          IDX16 ? reinterpret_cast<const chunk_t *>((const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base))
+                : reinterpret_cast<const chunk_t *>(p.plrep_j + pb);
       const int nch = (nr + E - 1) / E;
-      uint4 q = make_uint4(0u, 0u, 0u, 0u);
-      if (l < nch) q = row4[l];
+      chunk_t q = {};
+      if (l < nch) q = rowc[l];
       int jn = 0;
       float r0n = 0.0f;
       if (l < nc) {
@@ -1201,14 +1219,14 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       }
       // non-native repulsion: lane l takes chunks l, l + LPB, ... of the row,
       // one load per E pairs, the next chunk's load issued before the current
-      // chunk's gathers; entries past the count gather the bead itself and
-      // contribute nothing
+      // chunk's gathers; the row's padding entries are the bead itself and
+      // contribute nothing (d2 = 0 fails the guard)
       {
         const float s2A = S.sig2_rep[row], s2B = ONE ? s2A : S.sig2_rep[row + 1];
         const float m12A = S.rep_m12[row], m12B = ONE ? m12A : S.rep_m12[row + 1];
         const float m6A = S.rep_m6[row], m6B = ONE ? m6A : S.rep_m6[row + 1];
         const float gA = S.rep_guard[row], gB = ONE ? gA : S.rep_guard[row + 1];
-        auto rep = [&](float4 xj, bool ok) {
+        auto rep = [&](float4 xj) {
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const bool tb = !ONE && w_type(xj.w) == 2;
@@ -1217,21 +1235,22 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           const float q2 = s2 * ir2;
           const float q6 = q2 * q2 * q2;
           float c = ir2 * q6 * fmaf(tb ? m12B : m12A, q6, tb ? m6B : m6A);
-          c = ok && d2 > (tb ? gB : gA) ? c : 0.0f;
+          c = d2 > (tb ? gB : gA) ? c : 0.0f;
           fx = fmaf(c, dx, fx);
           fy = fmaf(c, dy, fy);
           fz = fmaf(c, dz, fz);
         };
         for (int c = l; c < nch; c += LPB) {
-          const uint4 cur = q;
-          if (c + LPB < nch) q = row4[c + LPB];
-          const uint32_t ww[4] = {cur.x, cur.y, cur.z, cur.w};
-#pragma unroll
-          for (int e = 0; e < E; ++e) {
-            const int j = IDX16 ? bead_off + (int)((ww[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu) : (int)ww[e];
-            const bool ok = c * E + e < nr;
-            rep(partner<LDS>(pin, s_pos, lds0, ok ? j : i), ok);
+          const chunk_t cur = q;
+          if (c + LPB < nch) q = rowc[c + LPB];
+          uint32_t ww[4];
+          if constexpr (IDX16) {
+            ww[0] = cur.x & 0xFFFFu; ww[1] = cur.x >> 16; ww[2] = cur.y & 0xFFFFu; ww[3] = cur.y >> 16;
+          } else {
+            ww[0] = cur.x; ww[1] = cur.y; ww[2] = cur.z; ww[3] = cur.w;
           }
+#pragma unroll
+          for (int e = 0; e < E; ++e) rep(partner<LDS>(pin, s_pos, lds0, IDX16 ? bead_off + (int)ww[e] : (int)ww[e]));
         }
       }
     }
