@@ -126,7 +126,11 @@ typedef struct sop_engine_opts {
   int32_t grid_blocks;    /* persistent grid size; 0 = 8 per compute unit */
   int32_t use_graphs;     /* capture step blocks as hipGraphs (default 1) */
   int32_t lds_beads_max;  /* systems up to this size run the step with LDS-resident
-                             coordinates (default 1024; < 0 disables) */
+                             coordinates (default 5056; < 0 disables) */
+  float pl_shell;         /* pair-list shell width in A (default 2; < 0 or > 8: none): the
+                             cell grid lists pairs within rc_pl + pl_shell, and later
+                             pair-list rebuilds filter those rows while every bead has
+                             moved less than pl_shell / 2 (same lists, less work) */
   int64_t pool_bytes;     /* serve: HBM budget for all pools; 0 = grow on demand */
 } sop_engine_opts;
 
@@ -209,6 +213,10 @@ sop_status sop_get_increments(sop_engine *e, int32_t sys, float *d3n);
 typedef struct sop_list_stats {
   int64_t nl_native, nl_rep, pl_native, pl_rep; /* half-list entry counts */
   int32_t nl_max_per_bead, pl_max_per_bead;
+  /* pair-list rebuilds since the system was added, by how they were made: from
+   * the cell grid (with new shell rows), by filtering the shell rows, or by
+   * filtering the materialized neighbor list */
+  int64_t grid_builds, shell_builds, nl_builds;
 } sop_list_stats;
 sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *st);
 /* Time `reps` launches of the fused force+integrate kernel over the listed

@@ -63,7 +63,7 @@ class EngineOpts(C.Structure):
     _fields_ = [
         ("nl_cap_max", C.c_int32), ("pl_cap_max", C.c_int32), ("cell_cap_max", C.c_int32),
         ("lanes_per_bead", C.c_int32), ("grid_blocks", C.c_int32), ("use_graphs", C.c_int32),
-        ("lds_beads_max", C.c_int32), ("pool_bytes", C.c_int64),
+        ("lds_beads_max", C.c_int32), ("pl_shell", C.c_float), ("pool_bytes", C.c_int64),
     ]
 
 
@@ -71,6 +71,7 @@ class ListStats(C.Structure):
     _fields_ = [
         ("nl_native", C.c_int64), ("nl_rep", C.c_int64), ("pl_native", C.c_int64), ("pl_rep", C.c_int64),
         ("nl_max_per_bead", C.c_int32), ("pl_max_per_bead", C.c_int32),
+        ("grid_builds", C.c_int64), ("shell_builds", C.c_int64), ("nl_builds", C.c_int64),
     ]
 
 

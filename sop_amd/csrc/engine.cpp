@@ -125,7 +125,7 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.cell_of, 4},    {(void **)&p.cell_tmp, 4},    {(void **)&p.cell_sorted, 4},
       {(void **)&p.cell_pos, 4},   {(void **)&p.nat_sorted, 8},
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
-      {(void **)&p.tile_ord, 1}};
+      {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},   {(void **)&p.nat_r0, 4},   {(void **)&p.nlatt_j, 4}, {(void **)&p.natslot, 8},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
@@ -190,6 +190,8 @@ static sop_status ensure_capacity(sop_engine *e, int64_t beads, int64_t nat, int
   if (pl > c.pl) {
     const int64_t nn = next(c.pl, pl);
     sop_status s = grow_array(e, (void **)&e->p.plrep_j, 4 * (size_t)c.pl, 4 * (size_t)nn);
+    if (s) return s;
+    s = grow_array(e, (void **)&e->p.slrep_j, 4 * (size_t)c.pl, 4 * (size_t)nn);
     if (s) return s;
     e->a_pl.grow(nn);
     c.pl = nn;
@@ -267,7 +269,7 @@ int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
   const int64_t cell_cap = std::min<int64_t>(e->opts.cell_cap_max, std::max<int64_t>(64, 2 * n)) + 1;
   int64_t per_bead = 4;  // tile table share
   for (const auto &a : e->bead_arrays) per_bead += (int64_t)a.second;
-  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + pl_cap) + 12 * cell_cap;
+  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + 2 * pl_cap) + 12 * cell_cap;
 }
 
 // Validate and upload one system, without its first lists (engine_init_lists).
@@ -500,6 +502,19 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   }
   S.cell_min_edge_pl = std::max(1.0e-3f, max_rcp * 1.001f);
   S.skin = skin > 0.0f ? skin * (1.0f - 1.0e-5f) : 0.0f;
+  // pair-list shell: rows within rc_pl + shell of pos_shell, valid while
+  // 2 max|x - pos_shell| < shell_ok (the shell less a 1e-4 margin, so the
+  // rounding of the shell cutoffs cannot matter)
+  const float shell = e->opts.pl_shell;
+  S.shell = shell;
+  S.shell_ok = shell * (1.0f - 1.0e-4f);
+  for (int q = 0; q < 4; ++q) {
+    const int ti = q / 2 + 1, tj = q % 2 + 1;
+    const double rs = (double)cutoff_host(P.pl_factor, P.pl_abs, P.sigma_rep[ti][tj]) + (double)shell;
+    S.rc2_sl_rep[q] = (float)(rs * rs);
+  }
+  S.cell_min_edge_sl = std::max(1.0e-3f, (max_rcp + shell) * 1.001f);
+  S.sl_valid = 0;
   S.build = BUILD_NONE;
   S.nl_valid = 0;
   int tmask = 0;
@@ -807,6 +822,7 @@ void sop_engine_opts_default(sop_engine_opts *o) {
   o->grid_blocks = 0;
   o->use_graphs = 1;
   o->lds_beads_max = 5056;  // two 1024-thread step blocks per compute unit at 80,896 B each
+  o->pl_shell = 2.0f;
   o->pool_bytes = 0;
 }
 
@@ -828,6 +844,8 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     if (e->opts.cell_cap_max <= 0) e->opts.cell_cap_max = def.cell_cap_max;
     if (e->opts.lanes_per_bead <= 0) e->opts.lanes_per_bead = def.lanes_per_bead;
     if (e->opts.lds_beads_max == 0) e->opts.lds_beads_max = def.lds_beads_max;
+    if (e->opts.pl_shell == 0.0f) e->opts.pl_shell = def.pl_shell;
+    if (!(e->opts.pl_shell > 0.0f) || e->opts.pl_shell > 8.0f) e->opts.pl_shell = 0.0f;
   }
   const int lpb = e->opts.lanes_per_bead;
   if (lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) {
@@ -912,7 +930,7 @@ void sop_engine_destroy(sop_engine *e) {
   for (auto &a : e->bead_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(bead pool)");
   for (auto &a : e->nat_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(native pool)");
   for (auto &a : e->cell_arrays) if (*a.first) note(hipFree(*a.first), "hipFree(cell pool)");
-  void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.tile_sys, e->p.sys,      e->p.sysout,
+  void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.slrep_j,  e->p.tile_sys, e->p.sys,  e->p.sysout,
                     e->p.syslist, e->p.work,    e->p.nwork,    e->p.cellwork, e->p.items};
   for (void *q : others) if (q) note(hipFree(q), "hipFree");
   if (e->stream) note(hipStreamDestroy(e->stream), "hipStreamDestroy");
@@ -1057,6 +1075,12 @@ sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *stt) {
   stt->pl_rep = tot[3] / 2;
   stt->nl_max_per_bead = mx[1];
   stt->pl_max_per_bead = mx[3];
+  SysDev S;
+  st = engine_read_sys(e, sys, &S);
+  if (st) return st;
+  stt->grid_builds = S.nbuild[0];
+  stt->shell_builds = S.nbuild[1];
+  stt->nl_builds = S.nbuild[2];
   return SOP_OK;
 }
 

@@ -187,6 +187,18 @@ __device__ __forceinline__ int cell_index(const SysDev &S, float4 x) {
 // edge >= rc_pl (BUILD_DIRECT) and the neighbor list need not be written.
 // Otherwise the neighbor list is materialized from pos_ref and filtered
 // (BUILD_NL).  Both give the same set as the reference's two-level scheme.
+//
+// The direct build writes shell rows: master pairs with d(x) < rc_pl + shell,
+// snapshot pos_shell = x, and the pair list is those rows filtered at x.  At a
+// later direct-capable rebuild where every bead has moved less than shell/2
+// since pos_shell, every pair with d(x) < rc_pl has d(pos_shell) < rc_pl +
+// shell, so filtering the same rows again gives the same pair list without
+// the cell grid (BUILD_SHELL).
+
+// Systems whose rebuild sorts beads into a cell grid.
+__device__ __forceinline__ bool grid_build(const SysDev &S) {
+  return S.build != BUILD_NONE && S.build != BUILD_SHELL;
+}
 
 __device__ __forceinline__ const float4 *build_src(const DevPtrs &p, const SysDev &S, int kstep) {
   return S.build == BUILD_DIRECT ? buf_in(p, S.parity + kstep) : p.pos_ref;
@@ -194,7 +206,8 @@ __device__ __forceinline__ const float4 *build_src(const DevPtrs &p, const SysDe
 
 // Per system: snapshot and build decision.
 __global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
-  __shared__ float red[4];
+  __shared__ float red[8];
+  __shared__ int s_build;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
@@ -204,7 +217,8 @@ __global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
     const bool nl = !mat && ((flags & FORCE_NL) || due_nl(S, kstep));
     const bool pl = !mat && ((flags & FORCE_PL) || due_pl(S, kstep));
     const float4 *x = buf_in(p, S.parity + kstep);
-    float d2max = 0.0f;
+    const bool shell_chk = pl && S.sl_valid;
+    float d2max = 0.0f, d2sh = 0.0f;
     for (int b = tid; b < S.n; b += blockDim.x) {
       const int i = S.bead_off + b;
       const float4 xi = x[i];
@@ -215,23 +229,48 @@ __global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
         const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
         d2max = fmaxf(d2max, dx * dx + dy * dy + dz * dz);
       }
+      if (shell_chk) {
+        const float4 r = p.pos_shell[i];
+        const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
+        d2sh = fmaxf(d2sh, dx * dx + dy * dy + dz * dz);
+      }
     }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) d2max = fmaxf(d2max, __shfl_xor(d2max, o));
-    if (lane == 0) red[wave] = d2max;
+    for (int o = 32; o >= 1; o >>= 1) {
+      d2max = fmaxf(d2max, __shfl_xor(d2max, o));
+      d2sh = fmaxf(d2sh, __shfl_xor(d2sh, o));
+    }
+    if (lane == 0) {
+      red[wave] = d2max;
+      red[4 + wave] = d2sh;
+    }
     __syncthreads();
     if (tid == 0) {
       const float dmax = sqrtf(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
+      const float dsh = sqrtf(fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7])));
       int build = BUILD_NONE;
       if (mat) {
         build = S.nl_valid ? BUILD_NONE : BUILD_NL_ONLY;
       } else {
         if (nl) S.nl_valid = 0;
-        if (pl) build = (S.skin > 0.0f && (nl || 2.0f * dmax < S.skin)) ? BUILD_DIRECT : BUILD_NL;
-        else if (flags & FORCE_NL) build = BUILD_NL_ONLY;
+        if (pl) {
+          if (S.skin > 0.0f && (nl || 2.0f * dmax < S.skin))
+            build = shell_chk && 2.0f * dsh < S.shell_ok ? BUILD_SHELL : BUILD_DIRECT;
+          else
+            build = BUILD_NL;
+        } else if (flags & FORCE_NL) {
+          build = BUILD_NL_ONLY;
+        }
       }
       S.build = build;
+      if (build == BUILD_DIRECT) S.sl_valid = 1;
+      if (build == BUILD_DIRECT || build == BUILD_SHELL || build == BUILD_NL)
+        ++S.nbuild[build == BUILD_DIRECT ? 0 : (build == BUILD_SHELL ? 1 : 2)];
+      s_build = build;
     }
+    __syncthreads();
+    if (s_build == BUILD_DIRECT)  // the shell rows about to be built belong to these coordinates
+      for (int b = tid; b < S.n; b += blockDim.x) p.pos_shell[S.bead_off + b] = x[S.bead_off + b];
     __syncthreads();
   }
 }
@@ -245,7 +284,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
-    if (S.build == BUILD_NONE) continue;
+    if (!grid_build(S)) continue;
     const float4 *src = build_src(p, S, kstep);
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int b = tid; b < S.n; b += blockDim.x) {
@@ -268,7 +307,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
     }
     __syncthreads();
     if (tid == 0) {
-      const float min_edge = S.build == BUILD_DIRECT ? S.cell_min_edge_pl : S.cell_min_edge;
+      const float min_edge = S.build == BUILD_DIRECT ? S.cell_min_edge_sl : S.cell_min_edge;
       float ext_max = 0.0f;
       int nc = 1;
       for (int d = 0; d < 3; ++d) {
@@ -302,7 +341,10 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
       if (S.boxl > 0.0f) {
         const int ok = ext_max < 0.5f * S.boxl - S.wrap_margin;
         S.wrap_free = ok;
-        if (!ok) atomicOr(&S.err, ERR_PBC);
+        if (!ok) {
+          atomicOr(&S.err, ERR_PBC);
+          S.sl_valid = 0;
+        }
       } else {
         S.wrap_free = 1;
       }
@@ -322,7 +364,7 @@ __global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build == BUILD_NONE || S.sort_lds) continue;
+    if (!grid_build(S) || S.sort_lds) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = cell_index(S, build_src(p, S, kstep)[i]);
@@ -340,7 +382,7 @@ __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
-    if (S.build == BUILD_NONE || S.sort_lds) continue;
+    if (!grid_build(S) || S.sort_lds) continue;
     const int nc = S.ncells, cb = S.cell_base;
     int carry = 0, used = 0;
     for (int c0 = 0; c0 < nc; c0 += 256) {
@@ -380,7 +422,7 @@ __global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build == BUILD_NONE || S.sort_lds) continue;
+    if (!grid_build(S) || S.sort_lds) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = p.cell_of[i];
@@ -400,7 +442,7 @@ __global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build == BUILD_NONE || S.sort_lds) continue;
+    if (!grid_build(S) || S.sort_lds) continue;
     const int i = t * TILE + threadIdx.x;
     if (i < S.bead_off + S.n) {
       const int c = p.cell_of[i];
@@ -517,7 +559,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_cell_sort_lds(DevPtrs p, int k
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
-    if (S.build == BUILD_NONE || !S.sort_lds) continue;
+    if (!grid_build(S) || !S.sort_lds) continue;
     const int n = S.n, nc = S.ncells, bo = S.bead_off, cb = S.cell_base;
     const float4 *src = build_src(p, S, kstep);
     for (int c = tid; c < nc; c += SORT_THREADS) s_cnt[c] = 0;
@@ -640,10 +682,10 @@ __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
   }
 }
 
-// Per bead of a system that builds its pair list from the cell grid: the
-// native pair list (natives within their pair cutoff), and for every native
-// entry the partner's cell and sorted position, which k_pl_cells turns into
-// candidate slots.  16 lanes per bead.
+// Per bead of a system that builds its pair list from the cell grid or the
+// shell rows: the native pair list (natives within their pair cutoff), and,
+// for a grid build, every native entry's partner cell and sorted position,
+// which k_pl_cells turns into candidate slots.  16 lanes per bead.
 __global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int gq = lane >> 4, gl = lane & 15;
@@ -653,7 +695,8 @@ __global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
     if (t < 0) continue;
     const int s = p.tile_sys[t];
     const SysDev &S = p.sys[s];
-    if (S.build != BUILD_DIRECT) continue;
+    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
+    const bool slots = S.build == BUILD_DIRECT;
     const float4 *x = buf_in(p, S.parity + kstep);
     const float boxl = S.boxl;
     const double plf = S.pl_factor;
@@ -674,7 +717,7 @@ __global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
         if (k < nn) {
           j = p.nat_j[ns + k];
           r0 = p.nat_r0[ns + k];
-          p.natslot[ns + k] = make_int2(p.cell_of[j], p.cell_pos[j]);
+          if (slots) p.natslot[ns + k] = make_int2(p.cell_of[j], p.cell_pos[j]);
           const float rc = cutoff_of(plf, pla, r0);
           pred = dist2_exact(xi, x[j], boxl) < rc * rc;
         }
@@ -691,8 +734,8 @@ __global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
   }
 }
 
-// Block per home cell: the pair lists of the cell's beads straight from the
-// pair-cutoff grid.  The bounds of the 9 neighbor rows and all their
+// Block per home cell: the shell rows of the cell's beads straight from the
+// shell-cutoff grid.  The bounds of the 9 neighbor rows and all their
 // candidates are staged in LDS together (CAND_MAX at a time); every home bead
 // of the cell tests them.  Each bead's list comes out in row order, then
 // cell-sorted order: the order of a per-bead scan of the same cells.
@@ -715,30 +758,23 @@ __device__ __forceinline__ void pad_row(const DevPtrs &p, const SysDev &S, int i
 }
 
 // One home bead against the nc staged candidates [b0, b0 + nc), two per lane
-// per pass with no branch in the loop.  Fast form (EXACT = false): the staged
-// arrays are padded to a multiple of 128 with unreachable coordinates, and the
-// home bead's bitmap over candidate slots already excludes itself, its chain
-// neighbours i +- 1, 2 and its native partners, so a candidate costs its loads,
-// the distance and one bit test.  EXACT = true (more candidates than the
-// bitmap holds): bounds, topology and a search of the sorted native partners
-// per candidate.  Appends the partners within the pair cutoff to out in
-// candidate order; returns the new count.
-template <bool PBC, bool EXACT>
+// per pass: the form for home cells with more candidates than the exclusion
+// bitmaps hold, so bounds, topology and a search of the sorted native partners
+// are tested per candidate.  Appends the partners within the pair cutoff to
+// out in candidate order; returns the new count.
+template <bool PBC>
 __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_cj, int nc, float4 xi, int ji,
-                                            int chi, float rcA, float rcB, float boxl, const uint32_t *excl,
-                                            const int32_t *nat_g, int nn, int32_t *out, uint16_t *out16, int bo,
-                                            int cnt, int pl_cap, int lane) {
+                                            int chi, float rcA, float rcB, float boxl, const int32_t *nat_g, int nn,
+                                            int32_t *out, uint16_t *out16, int bo, int cnt, int pl_cap, int lane) {
   for (int k0 = 0; k0 < nc; k0 += 128) {
     int jj[2];
     float4 xj[2];
-    uint32_t ew[2];
     bool pr[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int ci = EXACT ? min(k0 + u * 64 + lane, nc - 1) : k0 + u * 64 + lane;
+      const int ci = min(k0 + u * 64 + lane, nc - 1);
       jj[u] = s_cj[ci];
       xj[u] = s_cand[ci];
-      ew[u] = EXACT ? 0u : excl[ci >> 5] >> (ci & 31);
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
@@ -751,19 +787,12 @@ __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_c
         dz = dz - boxl * rintf(dz / boxl);
       }
       const float d2 = (dx * dx + dy * dy) + dz * dz;
-      bool keep = d2 < ((wj & 3) == 2 ? rcB : rcA);
-      if (EXACT) {
-        keep &= (k0 + u * 64 + lane < nc) & (j != ji) & !(((wj >> 2) == chi) & ((unsigned)(j - ji + 2) <= 4u));
-      } else {
-        keep &= (ew[u] & 1u) == 0u;
-      }
-      pr[u] = keep;
+      pr[u] = (d2 < ((wj & 3) == 2 ? rcB : rcA)) & (k0 + u * 64 + lane < nc) & (j != ji) &
+              !(((wj >> 2) == chi) & ((unsigned)(j - ji + 2) <= 4u));
     }
-    if (EXACT) {
 #pragma unroll
-      for (int u = 0; u < 2; ++u)
-        if (pr[u] && native_pair(nat_g, nn, jj[u])) pr[u] = false;
-    }
+    for (int u = 0; u < 2; ++u)
+      if (pr[u] && native_pair(nat_g, nn, jj[u])) pr[u] = false;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const uint64_t mk = __ballot(pr[u]);
@@ -780,6 +809,60 @@ __device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_c
   return cnt;
 }
 
+// Output row of one bead as a raw buffer of pl_cap entries: stores past the
+// row's end are dropped by the buffer bounds check (the count still grows,
+// which the caller reports as an overflow).
+// The caller's row address and size are wave-uniform; readfirstlane puts
+// the descriptor in scalar registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(void *row, int bytes) {
+  const uint64_t a = (uint64_t)row;
+  const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(a >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  return __builtin_amdgcn_make_buffer_rsrc((void *)u, (short)0, __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+
+// The fast form of the home-bead scan: the staged candidates are padded to a
+// multiple of 128 with coordinates no pair can reach, and the home bead's
+// bitmap over candidate slots already excludes itself, its chain neighbours
+// i +- 1, 2 and its native partners, so a candidate costs its loads, the
+// distance, one bit test and, when kept, one buffer store.  No minimum image:
+// a rebuild whose extent reaches boxl/2 - margin raises ERR_PBC, so every
+// separation here is below boxl/2, where the minimum image is the identity.
+// ONE: every bead has the home bead's type (one cutoff rcA).
+template <bool ONE, bool IDX16>
+__device__ __forceinline__ int pl_scan_fast(const float4 *s_cand, const int *s_cj, int ncp, float4 xi, float rcA,
+                                            float rcB, const uint32_t *excl, __amdgpu_buffer_rsrc_t row, int bo,
+                                            int lane) {
+  int cnt = 0;
+  const uint32_t lbit = 1u << (lane & 31);
+  for (int k0 = 0; k0 < ncp; k0 += 128) {
+    int jj[2];
+    float4 xj[2];
+    uint32_t ew[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int ci = k0 + u * 64 + lane;
+      jj[u] = s_cj[ci];
+      xj[u] = s_cand[ci];
+      ew[u] = excl[ci >> 5];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+      const float d2 = (dx * dx + dy * dy) + dz * dz;
+      const float rc = ONE ? rcA : ((__float_as_int(xj[u].w) & 3) == 2 ? rcB : rcA);
+      const bool keep = (d2 < rc) & ((ew[u] & lbit) == 0u);
+      const uint64_t mk = __ballot(keep);
+      if (keep) {
+        const int o = cnt + lanes_below(mk);
+        if (IDX16) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(jj[u] - bo), row, o * 2, 0, 0);
+        else __builtin_amdgcn_raw_buffer_store_b32((unsigned int)jj[u], row, o * 4, 0, 0);
+      }
+      cnt += __popcll(mk);
+    }
+  }
+  return cnt;
+}
 
 __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
   __shared__ float4 s_home[HOME_MAX];
@@ -800,8 +883,8 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
     const float boxl = S.boxl;
     const int pl_cap = S.pl_cap;
     const int64_t pl_base = S.pl_base;
-    const float rc2_11 = S.rc2_pl_rep[0], rc2_12 = S.rc2_pl_rep[1], rc2_21 = S.rc2_pl_rep[2],
-                rc2_22 = S.rc2_pl_rep[3];
+    const float rc2_11 = S.rc2_sl_rep[0], rc2_12 = S.rc2_sl_rep[1], rc2_21 = S.rc2_sl_rep[2],
+                rc2_22 = S.rc2_sl_rep[3];
     const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
     const int cx = c % d0, cy = (c / d0) % d1, cz = c / (d0 * d1);
     // bounds of the home cell and of the 3x3 neighbor rows (x-range cx-1..cx+1)
@@ -899,13 +982,18 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           const int ji = s_hj[h];
           const int wi = __float_as_int(xi.w), chi = wi >> 2;
           const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
-          int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
-          uint16_t *out16 = S.idx16 ? (uint16_t *)(p.plrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
-          const int cnt = boxl > 0.0f
-              ? pl_scan_home<true, false>(s_cand, s_cj, T, xi, ji, chi, rcA, rcB, boxl, s_excl[h], nullptr, 0, out,
-                                          out16, bo, 0, pl_cap, lane)
-              : pl_scan_home<false, false>(s_cand, s_cj, T, xi, ji, chi, rcA, rcB, boxl, s_excl[h], nullptr, 0, out,
-                                           out16, bo, 0, pl_cap, lane);
+          int32_t *out = p.slrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
+          uint16_t *out16 = S.idx16 ? (uint16_t *)(p.slrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
+          const int ncp = (T + 127) & ~127;
+          const bool one = S.type_mask != 3;
+          const float rc1 = S.type_mask == 2 ? rcB : rcA;
+          void *row = S.idx16 ? (void *)out16 : (void *)out;
+          const __amdgpu_buffer_rsrc_t rr = row_rsrc(row, pl_cap * (S.idx16 ? 2 : 4));
+          int cnt;
+          if (one) cnt = S.idx16 ? pl_scan_fast<true, true>(s_cand, s_cj, ncp, xi, rc1, rc1, s_excl[h], rr, bo, lane)
+                                 : pl_scan_fast<true, false>(s_cand, s_cj, ncp, xi, rc1, rc1, s_excl[h], rr, bo, lane);
+          else cnt = S.idx16 ? pl_scan_fast<false, true>(s_cand, s_cj, ncp, xi, rcA, rcB, s_excl[h], rr, bo, lane)
+                             : pl_scan_fast<false, false>(s_cand, s_cj, ncp, xi, rcA, rcB, s_excl[h], rr, bo, lane);
           if (lane == 0) s_hcnt[h] = cnt;
         }
       } else {
@@ -918,14 +1006,14 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
             const int ji = s_hj[h];
             const int wi = __float_as_int(xi.w), chi = wi >> 2;
             const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
-            int32_t *out = p.plrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
-            uint16_t *out16 = S.idx16 ? (uint16_t *)(p.plrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
+            int32_t *out = p.slrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
+            uint16_t *out16 = S.idx16 ? (uint16_t *)(p.slrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
             const int32_t *natg = p.nat_j + s_hns[h];
             const int cnt = boxl > 0.0f
-                ? pl_scan_home<true, true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, nullptr, natg, s_hnn[h],
-                                           out, out16, bo, s_hcnt[h], pl_cap, lane)
-                : pl_scan_home<false, true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, nullptr, natg, s_hnn[h],
-                                            out, out16, bo, s_hcnt[h], pl_cap, lane);
+                ? pl_scan_home<true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, natg, s_hnn[h], out, out16, bo,
+                                     s_hcnt[h], pl_cap, lane)
+                : pl_scan_home<false>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, natg, s_hnn[h], out, out16, bo,
+                                      s_hcnt[h], pl_cap, lane);
             if (lane == 0) s_hcnt[h] = cnt;
           }
           __syncthreads();
@@ -934,9 +1022,11 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
       __syncthreads();
       if (tid < nh) {
         const int j = s_hj[tid], cr = s_hcnt[tid];
-        p.plrep_cnt[j] = min(cr, pl_cap);
-        if (cr > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
-        for (int k = 0; k < ROW_PAD - 1; ++k) pad_row(p, S, j, min(cr, pl_cap), k);
+        p.slrep_cnt[j] = min(cr, pl_cap);
+        if (cr > pl_cap) {
+          atomicOr(&p.sys[s].err, ERR_PL_OVF);
+          p.sys[s].sl_valid = 0;  // truncated rows must not be filtered again
+        }
       }
       __syncthreads();
     }
@@ -1071,6 +1161,79 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
   }
 }
 
+// Pair list = shell rows filtered at x with the reference's exact predicate
+// (dist2_exact < rc_pl^2), for the grid and shell builds; 16 lanes per bead,
+// entries in shell-row order, rows padded for the step kernel.
+__global__ __launch_bounds__(256) void k_shell_filter(DevPtrs p, int kstep) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gq = lane >> 4, gl = lane & 15;
+  const int nwork = *p.nwork;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int t = p.work[w];
+    if (t < 0) continue;
+    const int s = p.tile_sys[t];
+    const SysDev &S = p.sys[s];
+    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
+    const float4 *x = buf_in(p, S.parity + kstep);
+    const float boxl = S.boxl;
+    const int bo = S.bead_off, pl_cap = S.pl_cap;
+    const bool idx16 = S.idx16 != 0;
+    const int bend = min(t * TILE + TILE, bo + S.n);
+    for (int q = 0; q < TILE / 16; ++q) {
+      const int i = t * TILE + q * 16 + wave * 4 + gq;
+      const bool ok = i < bend;
+      const int nr = ok ? p.slrep_cnt[i] : 0;
+      const float4 xi = ok ? x[i] : make_float4(0, 0, 0, 0);
+      const int ti = ok ? (__float_as_int(xi.w) & 3) : 1;
+      const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
+      const int64_t row = S.pl_base + (int64_t)(i - bo) * pl_cap;
+      const uint16_t *src16 = (const uint16_t *)(p.slrep_j + S.pl_base) + (row - S.pl_base);
+      uint16_t *dst16 = (uint16_t *)(p.plrep_j + S.pl_base) + (row - S.pl_base);
+      const int nmax = max(max(__shfl(nr, 0), __shfl(nr, 16)), max(__shfl(nr, 32), __shfl(nr, 48)));
+      int cnt = 0;
+      for (int k0 = 0; k0 < nmax; k0 += 32) {
+        // two entries per lane in flight
+        int jj[2];
+        bool pr[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int k = k0 + u * 16 + gl;
+          jj[u] = 0;
+          if (k < nr) jj[u] = idx16 ? bo + (int)src16[k] : p.slrep_j[row + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int k = k0 + u * 16 + gl;
+          pr[u] = false;
+          if (k < nr) {
+            const float4 xj = x[jj[u]];
+            pr[u] = dist2_exact(xi, xj, boxl) < (w_type(xj.w) == 2 ? rcB : rcA);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const uint32_t bits = (uint32_t)(__ballot(pr[u]) >> (16 * gq)) & 0xFFFFu;
+          if (pr[u]) {
+            const int o = cnt + __popc(bits & ((1u << gl) - 1u));
+            if (o < pl_cap) {
+              if (idx16) dst16[o] = (uint16_t)(jj[u] - bo);
+              else p.plrep_j[row + o] = jj[u];
+            }
+          }
+          cnt += __popc(bits);
+        }
+      }
+      if (ok) {
+        if (gl == 0) {
+          p.plrep_cnt[i] = min(cnt, pl_cap);
+          if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+        }
+        pad_row(p, S, i, min(cnt, pl_cap), gl);
+      }
+    }
+  }
+}
+
 // Per tile of a system whose pair list was rebuilt: the order in which the
 // step kernel's lane groups take the tile's beads, by decreasing pair count
 // (non-native + native), ties by index, so the beads that one round of lane
@@ -1083,7 +1246,7 @@ __global__ __launch_bounds__(256) void k_tile_order(DevPtrs p) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build != BUILD_DIRECT && S.build != BUILD_NL) continue;
+    if (S.build != BUILD_DIRECT && S.build != BUILD_NL && S.build != BUILD_SHELL) continue;
     const int b0 = t * TILE, i = b0 + lane;
     const int key = i < S.bead_off + S.n ? p.plrep_cnt[i] + p.platt_cnt[i] : -1;
     int rank = 0;
@@ -1620,6 +1783,7 @@ hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kste
   hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_natives, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_shell_filter, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);

@@ -60,6 +60,14 @@ struct SysDev {
   float cell_min_edge;    // >= largest neighbor-list cutoff, with a 0.1% margin
   float cell_min_edge_pl; // >= largest non-native pair-list cutoff, with a 0.1% margin
   float skin;             // smallest (rc_nl - rc_pl) over pair kinds, less a 1e-5 margin
+  // pair-list shell: shell rows hold the master pairs within rc_pl + shell of
+  // pos_shell; while every bead has moved less than shell_ok / 2 since then,
+  // the pair list is the shell rows filtered at x (BUILD_SHELL)
+  float shell, shell_ok;
+  float rc2_sl_rep[4];    // shell-row cutoffs squared, (rc_pl + shell)^2
+  float cell_min_edge_sl; // >= largest shell-row cutoff, with a 0.1% margin
+  int32_t sl_valid;       // 1: the shell rows match pos_shell
+  int32_t nbuild[3];      // pair-list rebuilds: grid (BUILD_DIRECT), shell, neighbor list
   float wrap_margin;
   // cell grid, written by k_bbox at each neighbor-list rebuild
   float org[3];
@@ -83,6 +91,7 @@ constexpr int BUILD_NONE = 0;    // no list work at this step
 constexpr int BUILD_DIRECT = 1;  // pair list straight from a pair-cutoff cell grid
 constexpr int BUILD_NL = 2;      // materialize the neighbor list from pos_ref, then filter
 constexpr int BUILD_NL_ONLY = 3; // materialize the neighbor list only
+constexpr int BUILD_SHELL = 4;   // pair list = shell rows filtered at x (no cell grid)
 
 // k_skin flags
 constexpr int FORCE_NL = 1;      // rebuild the neighbor list now (snapshot)
@@ -104,6 +113,8 @@ struct DevPtrs {
   int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list: row i at pl_base +
                                                          // (i - bead_off) * pl_cap, entries int32 arena
                                                          // indices, or uint16 local indices (idx16)
+  int32_t *slrep_j; int32_t *slrep_cnt;                  // shell rows, same layout as the pair list
+  float4 *pos_shell;                                     // coordinates at the last shell-row build
   int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
   int32_t *cell_pos;    // per bead: its position in the system's cell-sorted order
