@@ -105,6 +105,10 @@ int grid_items_far(const sop_engine *e) {
   return std::max(8, std::min(g, need));
 }
 
+ListGrids list_grids(const sop_engine *e) {
+  return ListGrids{grid_tiles(e), grid_sys(e), grid_items(e), grid_items_far(e), e->cur_lds};
+}
+
 } // namespace sopk
 
 #define HIPCK(expr)                                            \
@@ -537,7 +541,7 @@ sop_status engine_init_lists(sop_engine *e, const int32_t *slots, int32_t n, sop
   if (n <= 0) return SOP_OK;
   sop_status st = engine_set_batch(e, slots, n);
   if (st) return st;
-  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_NL | FORCE_PL, e->stream));
+  HIPCK(launch_lists(e->p, list_grids(e), 0, FORCE_NL | FORCE_PL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   for (int32_t q = 0; q < n; ++q) {
     status[q] = engine_check_errors(e, &slots[q], 1);
@@ -757,10 +761,10 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
   hipGraphExec_t exec;
   if (it == e->graphs.end()) {
     hipGraph_t graph;
-    const int gt = grid_tiles(e), gs = grid_sys(e), lpb = e->opts.lanes_per_bead;
+    const int lpb = e->opts.lanes_per_bead;
     HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < G; ++k) {
-      if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, gt, gs, k, 0, e->stream);
+      if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, list_grids(e), k, 0, e->stream);
       (void)launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), k, e->cur_lds, e->stream);
     }
     (void)launch_advance(e->p, G, e->stream);
@@ -779,7 +783,7 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
 // system is due for (the kernels check each system's own schedule).
 sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
   if (nsteps <= 0) return SOP_OK;
-  const int gt = grid_tiles(e), gs = grid_sys(e), lpb = e->opts.lanes_per_bead;
+  const int lpb = e->opts.lanes_per_bead;
   std::vector<int64_t> st;
   std::vector<int32_t> nnl, npl;
   for (int32_t s : e->h_syslist) {
@@ -793,7 +797,7 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
       any_nl |= ((st[q] + k) % nnl[q]) == 0;
       any_pl |= ((st[q] + k) % npl[q]) == 0;
     }
-    if (any_nl || any_pl) HIPCK(launch_lists(e->p, gt, gs, (int)k, 0, e->stream));
+    if (any_nl || any_pl) HIPCK(launch_lists(e->p, list_grids(e), (int)k, 0, e->stream));
     HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), (int)k, e->cur_lds, e->stream));
   }
   HIPCK(launch_advance(e->p, (int)nsteps, e->stream));
@@ -973,7 +977,7 @@ sop_status sop_update_neighbor_list(sop_engine *e, int32_t sys) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_NL, e->stream));
+  HIPCK(launch_lists(e->p, list_grids(e), 0, FORCE_NL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -982,7 +986,7 @@ sop_status sop_update_pair_list(sop_engine *e, int32_t sys) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, FORCE_PL, e->stream));
+  HIPCK(launch_lists(e->p, list_grids(e), 0, FORCE_PL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -991,7 +995,7 @@ sop_status sop_update_pair_list(sop_engine *e, int32_t sys) {
 static sop_status materialize_nl(sop_engine *e, int32_t sys) {
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_lists(e->p, grid_tiles(e), grid_sys(e), 0, MATERIALIZE_NL, e->stream));
+  HIPCK(launch_lists(e->p, list_grids(e), 0, MATERIALIZE_NL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }

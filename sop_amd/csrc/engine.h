@@ -18,7 +18,7 @@
 
 namespace sopk {
 
-hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int flags, hipStream_t st);
+hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st);
 hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int grid_far, int kstep, size_t lds,
                        hipStream_t st);
 hipError_t step_lds_limit(size_t bytes);

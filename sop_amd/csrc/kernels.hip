@@ -980,7 +980,7 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         for (int h = wave; h < nh; h += 4) {
           const float4 xi = s_home[h];
           const int ji = s_hj[h];
-          const int wi = __float_as_int(xi.w), chi = wi >> 2;
+          const int wi = __float_as_int(xi.w);
           const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
           int32_t *out = p.slrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
           uint16_t *out16 = S.idx16 ? (uint16_t *)(p.slrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
@@ -1161,84 +1161,12 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
   }
 }
 
-// Pair list = shell rows filtered at x with the reference's exact predicate
-// (dist2_exact < rc_pl^2), for the grid and shell builds; 16 lanes per bead,
-// entries in shell-row order, rows padded for the step kernel.
-__global__ __launch_bounds__(256) void k_shell_filter(DevPtrs p, int kstep) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gq = lane >> 4, gl = lane & 15;
-  const int nwork = *p.nwork;
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const int t = p.work[w];
-    if (t < 0) continue;
-    const int s = p.tile_sys[t];
-    const SysDev &S = p.sys[s];
-    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
-    const float4 *x = buf_in(p, S.parity + kstep);
-    const float boxl = S.boxl;
-    const int bo = S.bead_off, pl_cap = S.pl_cap;
-    const bool idx16 = S.idx16 != 0;
-    const int bend = min(t * TILE + TILE, bo + S.n);
-    for (int q = 0; q < TILE / 16; ++q) {
-      const int i = t * TILE + q * 16 + wave * 4 + gq;
-      const bool ok = i < bend;
-      const int nr = ok ? p.slrep_cnt[i] : 0;
-      const float4 xi = ok ? x[i] : make_float4(0, 0, 0, 0);
-      const int ti = ok ? (__float_as_int(xi.w) & 3) : 1;
-      const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
-      const int64_t row = S.pl_base + (int64_t)(i - bo) * pl_cap;
-      const uint16_t *src16 = (const uint16_t *)(p.slrep_j + S.pl_base) + (row - S.pl_base);
-      uint16_t *dst16 = (uint16_t *)(p.plrep_j + S.pl_base) + (row - S.pl_base);
-      const int nmax = max(max(__shfl(nr, 0), __shfl(nr, 16)), max(__shfl(nr, 32), __shfl(nr, 48)));
-      int cnt = 0;
-      for (int k0 = 0; k0 < nmax; k0 += 32) {
-        // two entries per lane in flight
-        int jj[2];
-        bool pr[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int k = k0 + u * 16 + gl;
-          jj[u] = 0;
-          if (k < nr) jj[u] = idx16 ? bo + (int)src16[k] : p.slrep_j[row + k];
-        }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int k = k0 + u * 16 + gl;
-          pr[u] = false;
-          if (k < nr) {
-            const float4 xj = x[jj[u]];
-            pr[u] = dist2_exact(xi, xj, boxl) < (w_type(xj.w) == 2 ? rcB : rcA);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const uint32_t bits = (uint32_t)(__ballot(pr[u]) >> (16 * gq)) & 0xFFFFu;
-          if (pr[u]) {
-            const int o = cnt + __popc(bits & ((1u << gl) - 1u));
-            if (o < pl_cap) {
-              if (idx16) dst16[o] = (uint16_t)(jj[u] - bo);
-              else p.plrep_j[row + o] = jj[u];
-            }
-          }
-          cnt += __popc(bits);
-        }
-      }
-      if (ok) {
-        if (gl == 0) {
-          p.plrep_cnt[i] = min(cnt, pl_cap);
-          if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
-        }
-        pad_row(p, S, i, min(cnt, pl_cap), gl);
-      }
-    }
-  }
-}
-
-// Per tile of a system whose pair list was rebuilt: the order in which the
-// step kernel's lane groups take the tile's beads, by decreasing pair count
-// (non-native + native), ties by index, so the beads that one round of lane
-// groups computes together run loops of similar length.  The forces do not
-// depend on the order: each bead is summed by one group the same way.
+// Per tile of a system whose pair list is rebuilt: the order in which the
+// shell filter's and the step kernel's lane groups take the tile's beads, by
+// decreasing row length (non-native + native), ties by index, so the beads
+// that one round of lane groups computes together run loops of similar
+// length.  Results do not depend on the order: each bead is computed by one
+// group the same way.
 __global__ __launch_bounds__(256) void k_tile_order(DevPtrs p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwork = *p.nwork;
@@ -1248,7 +1176,10 @@ __global__ __launch_bounds__(256) void k_tile_order(DevPtrs p) {
     const SysDev &S = p.sys[p.tile_sys[t]];
     if (S.build != BUILD_DIRECT && S.build != BUILD_NL && S.build != BUILD_SHELL) continue;
     const int b0 = t * TILE, i = b0 + lane;
-    const int key = i < S.bead_off + S.n ? p.plrep_cnt[i] + p.platt_cnt[i] : -1;
+    // grid and shell builds: shell-row counts (the pair lists are filtered
+    // after this); neighbor-list builds: the previous pair-list counts
+    const int nrep = S.build == BUILD_NL ? p.plrep_cnt[i] : p.slrep_cnt[i];
+    const int key = i < S.bead_off + S.n ? nrep + p.platt_cnt[i] : -1;
     int rank = 0;
     for (int j = 0; j < TILE; ++j) {
       const int kj = __shfl(key, j);
@@ -1519,6 +1450,126 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
   }
 }
 
+// Pair list of one tile's beads = their shell rows filtered at x with the
+// reference's exact predicate (d2 < rc_pl^2, squares summed in order without
+// contraction), for grid and shell builds.  4 lanes per bead, 16 beads per
+// round: lane l takes 4-entry chunks l, l + 4, ... of the row, kept entries
+// are compacted per chunk position, so a row comes out in (chunk pass,
+// position, lane) order; rows are padded for the step kernel.
+template <bool LDS, bool IDX16>
+__device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, int s, int t,
+                                            const float4 *__restrict__ pin, const float4 *s_pos, int lds0) {
+  using chunk_t = typename std::conditional<IDX16, uint2, uint4>::type;
+  const int lane = threadIdx.x & 63, g = lane >> 2, l = lane & 3;
+  const uint32_t below = (1u << l) - 1u;
+  const int bo = S.bead_off, pl_cap = S.pl_cap;
+  const int bend = min(t * TILE + TILE, bo + S.n);
+  // row lengths of the whole tile in one load, picked up by shuffles; beads
+  // in the tile's order (k_tile_order), slot q = 16 r + g
+  const int ordm = p.tile_ord[t * TILE + lane];
+  const int t_nr = t * TILE + lane < bend ? p.slrep_cnt[t * TILE + lane] : 0;
+  for (int r = 0; r < 4; ++r) {
+    const int li = __shfl(ordm, r * 16 + g);
+    const int i = t * TILE + li;
+    const bool act = i < bend;
+    const int self = act ? i : bend - 1;  // a bead of this tile: safe to gather
+    const int nr = __shfl(t_nr, li);
+    const int nch = (nr + 3) >> 2;
+    const float4 xi = partner<LDS>(pin, s_pos, lds0, self);
+    const int ti = __float_as_int(xi.w) & 3;
+    const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
+    const int64_t row = (int64_t)(self - bo) * pl_cap;  // entries from the segment start
+    const chunk_t *src = IDX16 ? reinterpret_cast<const chunk_t *>((const uint16_t *)(p.slrep_j + S.pl_base) + row)
+                               : reinterpret_cast<const chunk_t *>(p.slrep_j + S.pl_base + row);
+    uint16_t *dst16 = (uint16_t *)(p.plrep_j + S.pl_base) + row;
+    int32_t *dst32 = p.plrep_j + S.pl_base + row;
+    // chunks in batches of CB per lane, all loads of a batch issued before
+    // its stores (a load issued after a store waits for the store as well)
+    constexpr int CB = 6;
+    const int clast = max(nch - 1, 0);
+    int cnt = 0;
+    for (int m0 = 0; __ballot(l + 4 * m0 < nch) != 0; m0 += CB) {
+      chunk_t ch[CB];
+#pragma unroll
+      for (int b = 0; b < CB; ++b) ch[b] = src[min(l + 4 * (m0 + b), clast)];
+#pragma unroll
+      for (int b = 0; b < CB; ++b) {
+        const int c = l + 4 * (m0 + b);
+        if (__ballot(c < nch) == 0) break;
+        int jj[4];
+        if constexpr (IDX16) {
+          jj[0] = bo + (int)(ch[b].x & 0xFFFFu); jj[1] = bo + (int)(ch[b].x >> 16);
+          jj[2] = bo + (int)(ch[b].y & 0xFFFFu); jj[3] = bo + (int)(ch[b].y >> 16);
+        } else {
+          jj[0] = (int)ch[b].x; jj[1] = (int)ch[b].y; jj[2] = (int)ch[b].z; jj[3] = (int)ch[b].w;
+        }
+        bool ok[4];
+        float4 xj[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          ok[e] = 4 * c + e < nr;
+          xj[e] = partner<LDS>(pin, s_pos, lds0, ok[e] ? jj[e] : self);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // the reference's predicate without the minimum image, which is the
+          // identity here (a grid build whose extent reaches boxl/2 - margin
+          // raises ERR_PBC, and beads move less than the margin before the next)
+          const float dx = xj[e].x - xi.x, dy = xj[e].y - xi.y, dz = xj[e].z - xi.z;
+          const bool keep = ok[e] & (((dx * dx + dy * dy) + dz * dz) < (w_type(xj[e].w) == 2 ? rcB : rcA));
+          const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
+          if (keep) {
+            const int o = cnt + __popc(bits & below);
+            if (o < pl_cap) {
+              if (IDX16) dst16[o] = (uint16_t)(jj[e] - bo);
+              else dst32[o] = jj[e];
+            }
+          }
+          cnt += __popc(bits);
+        }
+      }
+    }
+    if (act) {
+      if (l == 0) {
+        p.plrep_cnt[i] = min(cnt, pl_cap);
+        if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+      }
+      pad_row(p, S, i, min(cnt, pl_cap), l);
+    }
+  }
+}
+
+// Shell filter over the step kernel's work items (same LDS staging): wave w
+// of the block takes tile first + w of its item.
+template <bool KLDS>
+__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep) {
+  extern __shared__ float4 s_pos[];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int w0 = KLDS ? 0 : *p.nitems;
+  const int w1 = KLDS ? *p.nitems : w0 + *p.nitems_far;
+  for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
+    const int4 it = p.items[w];
+    if (it.x < 0) continue;
+    if (KLDS) {
+      __syncthreads();
+      for (int b = tid; b < it.w; b += STEP_THREADS) {
+        const int a = it.z + b;
+        const SysDev &Sb = p.sys[p.tile_sys[a / TILE]];
+        s_pos[b] = buf_in(p, Sb.parity + kstep)[a];
+      }
+      __syncthreads();
+    }
+    if (wave >= it.y) continue;
+    const int t = it.x + wave;
+    const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
+    const SysDev &S = p.sys[s];
+    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
+    if (S.idx16) filter_tile<KLDS, true>(p, S, s, t, buf_in(p, S.parity + kstep), s_pos, KLDS ? it.z : 0);
+    else filter_tile<KLDS, false>(p, S, s, t, buf_in(p, S.parity + kstep), s_pos, KLDS ? it.z : 0);
+  }
+}
+
 // 8 waves per SIMD: two blocks per compute unit
 template <int LPB, int MODE>
 __global__ __launch_bounds__(STEP_THREADS, 8) void k_step(DevPtrs p, int kstep) {
@@ -1771,7 +1822,8 @@ __global__ void k_selftest(int *out, float *fout) {
 // One list-rebuild point for the current batch: snapshot / decision, cell
 // grid and counting sort, then whichever of the direct pair-list build or the
 // neighbor-list materialization + filter each system needs.
-hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kstep, int flags, hipStream_t st) {
+hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st) {
+  const int grid_tiles = lg.grid_tiles, grid_sys = lg.grid_sys;
   hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(256), 0, st, p, kstep, flags);
   hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cell_sort_lds, dim3(grid_sys), dim3(SORT_THREADS), 0, st, p, kstep);
@@ -1783,10 +1835,13 @@ hipError_t launch_lists(const DevPtrs &p, int grid_tiles, int grid_sys, int kste
   hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_natives, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_shell_filter, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
+  if (lg.grid_items > 0)
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(lg.grid_items), dim3(STEP_THREADS), lg.lds, st, p, kstep);
+  if (lg.grid_far > 0)
+    hipLaunchKernelGGL(k_shell_filter<false>, dim3(lg.grid_far), dim3(STEP_THREADS), 0, st, p, kstep);
   hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
   return SOPK_LAUNCH_CHECK();
 }
 
@@ -1826,6 +1881,9 @@ hipError_t step_lds_limit(size_t bytes) {
   SOPK_LDS_ATTR(4, 1) SOPK_LDS_ATTR(8, 1) SOPK_LDS_ATTR(16, 1) SOPK_LDS_ATTR(32, 1)
   SOPK_LDS_ATTR(4, 2) SOPK_LDS_ATTR(8, 2) SOPK_LDS_ATTR(16, 2) SOPK_LDS_ATTR(32, 2)
 #undef SOPK_LDS_ATTR
+  if (h == hipSuccess)
+    h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   return h;
 }
 

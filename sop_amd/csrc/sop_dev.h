@@ -144,6 +144,13 @@ struct DevPtrs {
 constexpr int STEP_WAVES = 16;
 constexpr int STEP_THREADS = 64 * STEP_WAVES;
 
+// Grids of the list kernels: per tile, per system, and the step kernel's
+// work items (the shell filter shares its LDS staging).
+struct ListGrids {
+  int grid_tiles, grid_sys, grid_items, grid_far;
+  size_t lds;
+};
+
 // systems whose cell sort fits one block's LDS
 constexpr int SORT_N_MAX = 5120;
 constexpr int SORT_C_MAX = 4096;
