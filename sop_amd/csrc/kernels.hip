@@ -1462,8 +1462,14 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   using chunk_t = typename std::conditional<IDX16, uint2, uint4>::type;
   const int lane = threadIdx.x & 63, g = lane >> 2, l = lane & 3;
   const uint32_t below = (1u << l) - 1u;
+  // system constants in registers: the stores below may alias SysDev as far
+  // as the compiler knows, and a reload would wait for every earlier store
   const int bo = S.bead_off, pl_cap = S.pl_cap;
+  const int64_t pl_base = S.pl_base;
   const int bend = min(t * TILE + TILE, bo + S.n);
+  // cutoffs as wave-uniform values (a per-lane indexed load would be a vector
+  // load, and waiting for it would also wait for the earlier stores)
+  const float rc11 = S.rc2_pl_rep[0], rc12 = S.rc2_pl_rep[1], rc21 = S.rc2_pl_rep[2], rc22 = S.rc2_pl_rep[3];
   // row lengths of the whole tile in one load, picked up by shuffles; beads
   // in the tile's order (k_tile_order), slot q = 16 r + g
   const int ordm = p.tile_ord[t * TILE + lane];
@@ -1476,13 +1482,13 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     const int nr = __shfl(t_nr, li);
     const int nch = (nr + 3) >> 2;
     const float4 xi = partner<LDS>(pin, s_pos, lds0, self);
-    const int ti = __float_as_int(xi.w) & 3;
-    const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
+    const bool t2 = (__float_as_int(xi.w) & 3) == 2;
+    const float rcA = t2 ? rc21 : rc11, rcB = t2 ? rc22 : rc12;
     const int64_t row = (int64_t)(self - bo) * pl_cap;  // entries from the segment start
-    const chunk_t *src = IDX16 ? reinterpret_cast<const chunk_t *>((const uint16_t *)(p.slrep_j + S.pl_base) + row)
-                               : reinterpret_cast<const chunk_t *>(p.slrep_j + S.pl_base + row);
-    uint16_t *dst16 = (uint16_t *)(p.plrep_j + S.pl_base) + row;
-    int32_t *dst32 = p.plrep_j + S.pl_base + row;
+    const chunk_t *src = IDX16 ? reinterpret_cast<const chunk_t *>((const uint16_t *)(p.slrep_j + pl_base) + row)
+                               : reinterpret_cast<const chunk_t *>(p.slrep_j + pl_base + row);
+    uint16_t *dst16 = (uint16_t *)(p.plrep_j + pl_base) + row;
+    int32_t *dst32 = p.plrep_j + pl_base + row;
     // chunks in batches of CB per lane, all loads of a batch issued before
     // its stores (a load issued after a store waits for the store as well)
     constexpr int CB = 6;
@@ -1530,11 +1536,16 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
       }
     }
     if (act) {
+      const int c = min(cnt, pl_cap);
       if (l == 0) {
-        p.plrep_cnt[i] = min(cnt, pl_cap);
+        p.plrep_cnt[i] = c;
         if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
       }
-      pad_row(p, S, i, min(cnt, pl_cap), l);
+      // pad to a multiple of ROW_PAD with the bead itself (pad_row)
+      if (l < ((ROW_PAD - (c & (ROW_PAD - 1))) & (ROW_PAD - 1)) && c + l < pl_cap) {
+        if (IDX16) dst16[c + l] = (uint16_t)(i - bo);
+        else dst32[c + l] = i;
+      }
     }
   }
 }
