@@ -42,8 +42,12 @@ def algorithmic_bytes(eng, ids, systems, n_pl, n_nl):
     step = 0.0
     full = 0.0
     beads = 0
+    builds = [0, 0, 0]
     for i, s in zip(ids, systems):
         st = eng.list_stats(i)
+        builds[0] += st.grid_builds
+        builds[1] += st.shell_builds
+        builds[2] += st.nl_builds
         n = s.n
         b = n - 1
         pa, pr, qa, qr = st.pl_native, st.pl_rep, st.nl_native, st.nl_rep
@@ -51,6 +55,8 @@ def algorithmic_bytes(eng, ids, systems, n_pl, n_nl):
         step += f
         full += f + (12 * n + 8 * qa + 4 * qr + 8 * pa + 4 * pr) / n_pl + (28 * n + 8 * qa + 4 * qr) / n_nl
         beads += n
+    log(f"pair-list rebuilds over all systems: {builds[0]} from the cell grid, {builds[1]} from shell rows, "
+        f"{builds[2]} from the neighbor list")
     return step, full / beads
 
 
@@ -126,6 +132,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--lds-beads", type=int, default=0, help="LDS-resident step for systems up to this size "
                     "(0 = engine default, < 0 = off)")
+    ap.add_argument("--pl-shell", type=float, default=None,
+                    help="pair-list shell width in A (default: the engine's, 3 A; < 0 = none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration for profiles)")
     args = ap.parse_args()
@@ -145,7 +153,7 @@ def main():
     beads = sum(s.n for s in systems)
     log(f"workload: {len(systems)} replicas, {beads} beads, generated in {time.perf_counter() - t:.1f}s")
 
-    eng = S.Engine(0, lanes_per_bead=args.lanes, lds_beads_max=args.lds_beads)
+    eng = S.Engine(0, lanes_per_bead=args.lanes, lds_beads_max=args.lds_beads, pl_shell=args.pl_shell)
     t = time.perf_counter()
     ids = [eng.add(s) for s in systems]
     log(f"resident on device in {time.perf_counter() - t:.1f}s")

@@ -127,7 +127,7 @@ typedef struct sop_engine_opts {
   int32_t use_graphs;     /* capture step blocks as hipGraphs (default 1) */
   int32_t lds_beads_max;  /* systems up to this size run the step with LDS-resident
                              coordinates (default 5056; < 0 disables) */
-  float pl_shell;         /* pair-list shell width in A (default 2; < 0 or > 8: none): the
+  float pl_shell;         /* pair-list shell width in A (default 3; < 0 or > 8: none): the
                              cell grid lists pairs within rc_pl + pl_shell, and later
                              pair-list rebuilds filter those rows while every bead has
                              moved less than pl_shell / 2 (same lists, less work) */

@@ -826,7 +826,7 @@ void sop_engine_opts_default(sop_engine_opts *o) {
   o->grid_blocks = 0;
   o->use_graphs = 1;
   o->lds_beads_max = 5056;  // two 1024-thread step blocks per compute unit at 80,896 B each
-  o->pl_shell = 2.0f;
+  o->pl_shell = 3.0f;
   o->pool_bytes = 0;
 }
 
