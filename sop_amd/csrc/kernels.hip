@@ -205,8 +205,10 @@ __device__ __forceinline__ const float4 *build_src(const DevPtrs &p, const SysDe
 }
 
 // Per system: snapshot and build decision.
-__global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
-  __shared__ float red[8];
+constexpr int SKIN_THREADS = 1024;
+__global__ __launch_bounds__(SKIN_THREADS) void k_skin(DevPtrs p, int kstep, int flags) {
+  constexpr int NW = SKIN_THREADS / 64;
+  __shared__ float red[2][NW];
   __shared__ int s_build;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
@@ -216,23 +218,27 @@ __global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
     const bool mat = (flags & MATERIALIZE_NL) != 0;
     const bool nl = !mat && ((flags & FORCE_NL) || due_nl(S, kstep));
     const bool pl = !mat && ((flags & FORCE_PL) || due_pl(S, kstep));
+    // system constants in registers (the snapshot stores below may alias S)
+    const int bo = S.bead_off, n = S.n;
     const float4 *x = buf_in(p, S.parity + kstep);
     const bool shell_chk = pl && S.sl_valid;
     float d2max = 0.0f, d2sh = 0.0f;
-    for (int b = tid; b < S.n; b += blockDim.x) {
-      const int i = S.bead_off + b;
-      const float4 xi = x[i];
-      if (nl) {
-        p.pos_ref[i] = xi;
-      } else if (pl) {
-        const float4 r = p.pos_ref[i];
-        const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
-        d2max = fmaxf(d2max, dx * dx + dy * dy + dz * dz);
-      }
-      if (shell_chk) {
-        const float4 r = p.pos_shell[i];
-        const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
-        d2sh = fmaxf(d2sh, dx * dx + dy * dy + dz * dz);
+    if (nl || pl) {
+      for (int b = tid; b < n; b += SKIN_THREADS) {
+        const int i = bo + b;
+        const float4 xi = x[i];
+        if (nl) {
+          p.pos_ref[i] = xi;
+        } else {
+          const float4 r = p.pos_ref[i];
+          const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
+          d2max = fmaxf(d2max, dx * dx + dy * dy + dz * dz);
+        }
+        if (shell_chk) {
+          const float4 r = p.pos_shell[i];
+          const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
+          d2sh = fmaxf(d2sh, dx * dx + dy * dy + dz * dz);
+        }
       }
     }
 #pragma unroll
@@ -241,13 +247,17 @@ __global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
       d2sh = fmaxf(d2sh, __shfl_xor(d2sh, o));
     }
     if (lane == 0) {
-      red[wave] = d2max;
-      red[4 + wave] = d2sh;
+      red[0][wave] = d2max;
+      red[1][wave] = d2sh;
     }
     __syncthreads();
     if (tid == 0) {
-      const float dmax = sqrtf(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
-      const float dsh = sqrtf(fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7])));
+      float m0 = 0.0f, m1 = 0.0f;
+      for (int w = 0; w < NW; ++w) {
+        m0 = fmaxf(m0, red[0][w]);
+        m1 = fmaxf(m1, red[1][w]);
+      }
+      const float dmax = sqrtf(m0), dsh = sqrtf(m1);
       int build = BUILD_NONE;
       if (mat) {
         build = S.nl_valid ? BUILD_NONE : BUILD_NL_ONLY;
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(256) void k_skin(DevPtrs p, int kstep, int flags) {
     }
     __syncthreads();
     if (s_build == BUILD_DIRECT)  // the shell rows about to be built belong to these coordinates
-      for (int b = tid; b < S.n; b += blockDim.x) p.pos_shell[S.bead_off + b] = x[S.bead_off + b];
+      for (int b = tid; b < n; b += SKIN_THREADS) p.pos_shell[bo + b] = x[bo + b];
     __syncthreads();
   }
 }
@@ -682,10 +692,9 @@ __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
   }
 }
 
-// Per bead of a system that builds its pair list from the cell grid or the
-// shell rows: the native pair list (natives within their pair cutoff), and,
-// for a grid build, every native entry's partner cell and sorted position,
-// which k_pl_cells turns into candidate slots.  16 lanes per bead.
+// Per native CSR entry of a system that builds its shell rows from the cell
+// grid: the partner's cell and sorted position, which k_pl_cells turns into
+// candidate slots.  16 lanes per bead.
 __global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int gq = lane >> 4, gl = lane & 15;
@@ -695,41 +704,16 @@ __global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
     if (t < 0) continue;
     const int s = p.tile_sys[t];
     const SysDev &S = p.sys[s];
-    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
-    const bool slots = S.build == BUILD_DIRECT;
-    const float4 *x = buf_in(p, S.parity + kstep);
-    const float boxl = S.boxl;
-    const double plf = S.pl_factor;
-    const float pla = S.pl_abs;
+    if (S.build != BUILD_DIRECT) continue;
     const int bend = min(t * TILE + TILE, S.bead_off + S.n);
     for (int q = 0; q < TILE / 16; ++q) {
       const int i = t * TILE + q * 16 + wave * 4 + gq;
-      const bool ok = i < bend;
-      const int ns = ok ? p.nat_start[i] : 0, nn = ok ? p.nat_cnt[i] : 0;
-      const float4 xi = ok ? x[i] : make_float4(0, 0, 0, 0);
-      int cnt = 0;
-      const int nmax = max(max(__shfl(nn, 0), __shfl(nn, 16)), max(__shfl(nn, 32), __shfl(nn, 48)));
-      for (int k0 = 0; k0 < nmax; k0 += 16) {
-        const int k = k0 + gl;
-        bool pred = false;
-        int j = 0;
-        float r0 = 0.0f;
-        if (k < nn) {
-          j = p.nat_j[ns + k];
-          r0 = p.nat_r0[ns + k];
-          if (slots) p.natslot[ns + k] = make_int2(p.cell_of[j], p.cell_pos[j]);
-          const float rc = cutoff_of(plf, pla, r0);
-          pred = dist2_exact(xi, x[j], boxl) < rc * rc;
-        }
-        const uint32_t bits = (uint32_t)(__ballot(pred) >> (16 * gq)) & 0xFFFFu;
-        if (pred) {
-          const int o = ns + cnt + __popc(bits & ((1u << gl) - 1u));
-          p.platt_j[o] = j;
-          p.platt_r0[o] = r0;
-        }
-        cnt += __popc(bits);
+      if (i >= bend) continue;
+      const int ns = p.nat_start[i], nn = p.nat_cnt[i];
+      for (int k = gl; k < nn; k += 16) {
+        const int j = p.nat_j[ns + k];
+        p.natslot[ns + k] = make_int2(p.cell_of[j], p.cell_pos[j]);
       }
-      if (ok && gl == 0) p.platt_cnt[i] = cnt;
     }
   }
 }
@@ -1452,10 +1436,11 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
 
 // Pair list of one tile's beads = their shell rows filtered at x with the
 // reference's exact predicate (d2 < rc_pl^2, squares summed in order without
-// contraction), for grid and shell builds.  4 lanes per bead, 16 beads per
-// round: lane l takes 4-entry chunks l, l + 4, ... of the row, kept entries
-// are compacted per chunk position, so a row comes out in (chunk pass,
-// position, lane) order; rows are padded for the step kernel.
+// contraction), for grid and shell builds, and the native pair list from the
+// native CSR.  4 lanes per bead, 16 beads per round: lane l takes 4-entry
+// chunks l, l + 4, ... of the row, kept entries are compacted per chunk
+// position, so a row comes out in (chunk pass, position, lane) order; rows are
+// padded for the step kernel.
 template <bool LDS, bool IDX16>
 __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, int s, int t,
                                             const float4 *__restrict__ pin, const float4 *s_pos, int lds0) {
@@ -1473,13 +1458,35 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   // row lengths of the whole tile in one load, picked up by shuffles; beads
   // in the tile's order (k_tile_order), slot q = 16 r + g
   const int ordm = p.tile_ord[t * TILE + lane];
-  const int t_nr = t * TILE + lane < bend ? p.slrep_cnt[t * TILE + lane] : 0;
+  const bool tin = t * TILE + lane < bend;
+  const int t_nr = tin ? p.slrep_cnt[t * TILE + lane] : 0;
+  const int t_ns = tin ? p.nat_start[t * TILE + lane] : 0, t_nn = tin ? p.nat_cnt[t * TILE + lane] : 0;
+  const double plf = S.pl_factor;
+  const float pla = S.pl_abs;
   for (int r = 0; r < 4; ++r) {
     const int li = __shfl(ordm, r * 16 + g);
     const int i = t * TILE + li;
     const bool act = i < bend;
     const int self = act ? i : bend - 1;  // a bead of this tile: safe to gather
     const int nr = __shfl(t_nr, li);
+    // shuffles outside any condition: a lane that a condition turns off does
+    // not provide its value to the lanes reading from it
+    const int ns = __shfl(t_ns, li), nn_all = __shfl(t_nn, li);
+    const int nn = act ? nn_all : 0;
+    // the bead's first NB natives per lane, loaded with the row's first batch
+    constexpr int NB = 4;
+    int nj[NB];
+    float nr0[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int k = l + 4 * b;
+      nj[b] = self;
+      nr0[b] = 0.0f;
+      if (k < nn) {
+        nj[b] = p.nat_j[ns + k];
+        nr0[b] = p.nat_r0[ns + k];
+      }
+    }
     const int nch = (nr + 3) >> 2;
     const float4 xi = partner<LDS>(pin, s_pos, lds0, self);
     const bool t2 = (__float_as_int(xi.w) & 3) == 2;
@@ -1534,6 +1541,42 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
           cnt += __popc(bits);
         }
       }
+    }
+    // native pair list: natives within their pair cutoff, in native order
+    {
+      int ncnt = 0;
+      for (int m0 = 0; __ballot(l + 4 * m0 < nn) != 0; m0 += NB) {
+        if (m0 > 0) {
+          // beads with more than 4 NB natives (rare): the next batch here
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const int k = l + 4 * (m0 + b);
+            nj[b] = self;
+            nr0[b] = 0.0f;
+            if (k < nn) {
+              nj[b] = p.nat_j[ns + k];
+              nr0[b] = p.nat_r0[ns + k];
+            }
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int k = l + 4 * (m0 + b);
+          const float4 xj = partner<LDS>(pin, s_pos, lds0, nj[b]);
+          const float rc = cutoff_of(plf, pla, nr0[b]);
+          // the reference's predicate without the minimum image (see above)
+          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+          const bool keep = (k < nn) & (((dx * dx + dy * dy) + dz * dz) < rc * rc);
+          const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
+          if (keep) {
+            const int o = ns + ncnt + __popc(bits & below);
+            p.platt_j[o] = nj[b];
+            p.platt_r0[o] = nr0[b];
+          }
+          ncnt += __popc(bits);
+        }
+      }
+      if (act && l == 0) p.platt_cnt[i] = ncnt;
     }
     if (act) {
       const int c = min(cnt, pl_cap);
@@ -1835,7 +1878,7 @@ __global__ void k_selftest(int *out, float *fout) {
 // neighbor-list materialization + filter each system needs.
 hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st) {
   const int grid_tiles = lg.grid_tiles, grid_sys = lg.grid_sys;
-  hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(256), 0, st, p, kstep, flags);
+  hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(SKIN_THREADS), 0, st, p, kstep, flags);
   hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cell_sort_lds, dim3(grid_sys), dim3(SORT_THREADS), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
