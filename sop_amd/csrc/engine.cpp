@@ -105,8 +105,21 @@ int grid_items_far(const sop_engine *e) {
   return std::max(8, std::min(g, need));
 }
 
+// Whole-CU LDS items: one block per compute unit (the block takes all of
+// the unit's LDS).
+int grid_items_big(const sop_engine *e) {
+  const int g = std::max(8, e->num_cu / 8 * 8);
+  if (e->cur_nitems_big == 0) return 0;  // no launch (always while serving)
+  const int need = (e->cur_nitems_big + 7) / 8 * 8;
+  return std::max(8, std::min(g, need));
+}
+
+StepGrids step_grids(const sop_engine *e) {
+  return StepGrids{grid_items(e), grid_items_big(e), grid_items_far(e), e->cur_lds, e->cur_lds_big};
+}
+
 ListGrids list_grids(const sop_engine *e) {
-  return ListGrids{grid_tiles(e), grid_sys(e), grid_items(e), grid_items_far(e), e->cur_lds};
+  return ListGrids{grid_tiles(e), grid_sys(e), step_grids(e)};
 }
 
 } // namespace sopk
@@ -610,11 +623,18 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   const int64_t lds_cap = e->opts.lds_beads_max > 0
                               ? std::min<int64_t>(e->opts.lds_beads_max, (int64_t)(e->lds_limit / 16))
                               : 0;
+  // systems above lds_cap that fit one compute unit's whole LDS take the
+  // whole-CU tier (one block per unit) instead of global gathers.  Not while
+  // serving: there every tier is launched at every step (fixed grids), and
+  // each launch costs at least one tile's latency in series with the others,
+  // which measured more than the tier saves on a served job mix.
+  const int64_t big_cap = e->fixed_grids ? 0 : e->lds_big_cap;
   std::vector<int32_t> by_arena(slots, slots + nslots);
   std::sort(by_arena.begin(), by_arena.end(),
             [&](int32_t a2, int32_t b2) { return e->sys[a2].bead_off < e->sys[b2].bead_off; });
-  std::vector<int4> lds_items;
+  std::vector<int4> lds_items, big_items;
   std::vector<std::vector<int4>> far_items;  // per non-LDS system
+  size_t lds_big = 0;
   int4 cur = make_int4(-1, 0, -1, 0);
   auto flush = [&]() {
     if (cur.x >= 0) lds_items.push_back(cur);
@@ -641,6 +661,11 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
         flush();
         cur = make_int4(t0, nt, (int32_t)h.bead_off, (int32_t)h.bead_len);
       }
+    } else if (h.bead_len <= big_cap) {
+      flush();
+      lds_big = std::max(lds_big, (size_t)h.bead_len * 16);
+      for (int32_t k = 0; k < nt; k += STEP_WAVES)
+        big_items.push_back(make_int4(t0 + k, std::min(STEP_WAVES, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
     } else {
       flush();
       std::vector<int4> v;
@@ -651,13 +676,19 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   flush();
   // XCD bins by tile load (item k runs on XCD k mod 8): the LDS items, then
   // whole non-LDS systems (largest first) so each keeps its gathers in one L2
-  std::vector<std::vector<int4>> ibins(NX), fbins(NX);
-  std::vector<int64_t> iload(NX, 0), fload(NX, 0);
+  std::vector<std::vector<int4>> ibins(NX), bbins(NX), fbins(NX);
+  std::vector<int64_t> iload(NX, 0), bload(NX, 0), fload(NX, 0);
   std::stable_sort(lds_items.begin(), lds_items.end(), [](const int4 &u, const int4 &v) { return u.y > v.y; });
   for (const int4 &it : lds_items) {
     const int x = (int)(std::min_element(iload.begin(), iload.end()) - iload.begin());
     ibins[x].push_back(it);
     iload[x] += it.y;
+  }
+  std::stable_sort(big_items.begin(), big_items.end(), [](const int4 &u, const int4 &v) { return u.y > v.y; });
+  for (const int4 &it : big_items) {
+    const int x = (int)(std::min_element(bload.begin(), bload.end()) - bload.begin());
+    bbins[x].push_back(it);
+    bload[x] += it.y;
   }
   std::sort(far_items.begin(), far_items.end(), [](const std::vector<int4> &u, const std::vector<int4> &v) {
     return u.size() != v.size() ? u.size() > v.size() : u[0].x < v[0].x;
@@ -669,17 +700,21 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
       fload[x] += it.y;
     }
   }
-  size_t imx = 0, fmx = 0;
+  size_t imx = 0, bmx = 0, fmx = 0;
   for (auto &bn : ibins) imx = std::max(imx, bn.size());
+  for (auto &bn : bbins) bmx = std::max(bmx, bn.size());
   for (auto &bn : fbins) fmx = std::max(fmx, bn.size());
-  e->h_items.assign((imx + fmx) * NX, make_int4(-1, 0, -1, 0));
+  e->h_items.assign((imx + bmx + fmx) * NX, make_int4(-1, 0, -1, 0));
   for (int x = 0; x < NX; ++x) {
     for (size_t m = 0; m < ibins[x].size(); ++m) e->h_items[m * NX + x] = ibins[x][m];
-    for (size_t m = 0; m < fbins[x].size(); ++m) e->h_items[(imx + m) * NX + x] = fbins[x][m];
+    for (size_t m = 0; m < bbins[x].size(); ++m) e->h_items[(imx + m) * NX + x] = bbins[x][m];
+    for (size_t m = 0; m < fbins[x].size(); ++m) e->h_items[(imx + bmx + m) * NX + x] = fbins[x][m];
   }
-  const int32_t n_lds_items = (int32_t)(imx * NX), n_far_items = (int32_t)(fmx * NX);
+  const int32_t n_lds_items = (int32_t)(imx * NX), n_big_items = (int32_t)(bmx * NX),
+                n_far_items = (int32_t)(fmx * NX);
   if (e->fixed_grids && lds_cap > 0) lds = (size_t)lds_cap * 16;
   e->cur_lds = lds;
+  e->cur_lds_big = lds_big;
   e->h_syslist.assign(slots, slots + nslots);
   if ((int64_t)e->h_work.size() + 2 > e->work_cap) {
     HIPCK(hipStreamSynchronize(e->stream));
@@ -695,6 +730,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->p.ncellwork = e->p.nwork + 32;
     e->p.nitems = e->p.nwork + 48;
     e->p.nitems_far = e->p.nwork + 56;
+    e->p.nitems_big = e->p.nwork + 60;
     invalidate_graphs(e);
   }
   if ((int64_t)e->h_items.size() + 1 > e->items_cap) {
@@ -712,6 +748,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   const int32_t ni = (int32_t)e->h_items.size();
   if (ni > 0) HIPCK(hipMemcpy(e->p.items, e->h_items.data(), sizeof(int4) * (size_t)ni, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nitems_far, &n_far_items, 4, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.nitems_big, &n_big_items, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nitems, &n_lds_items, 4, hipMemcpyHostToDevice));
@@ -719,6 +756,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   e->cur_nsys = ns;
   e->cur_nitems = n_lds_items;
   e->cur_nitems_far = n_far_items;
+  e->cur_nitems_big = n_big_items;
   return SOP_OK;
 }
 
@@ -746,7 +784,8 @@ sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslo
 static std::string graph_key(const sop_engine *e, int G, int n_nl, int n_pl) {
   std::ostringstream os;
   os << G << '|' << n_nl << '|' << n_pl << '|' << e->opts.lanes_per_bead << '|' << grid_tiles(e) << '|'
-     << grid_sys(e) << '|' << grid_items(e) << '|' << grid_items_far(e) << '|' << e->cur_lds;
+     << grid_sys(e) << '|' << grid_items(e) << '|' << grid_items_big(e) << '|' << grid_items_far(e) << '|'
+     << e->cur_lds << '|' << e->cur_lds_big;
   return os.str();
 }
 
@@ -765,7 +804,7 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
     HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < G; ++k) {
       if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, list_grids(e), k, 0, e->stream);
-      (void)launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), k, e->cur_lds, e->stream);
+      (void)launch_step(e->p, MODE_STEP, lpb, step_grids(e), k, e->stream);
     }
     (void)launch_advance(e->p, G, e->stream);
     HIPCK(hipStreamEndCapture(e->stream, &graph));
@@ -798,7 +837,7 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
       any_pl |= ((st[q] + k) % npl[q]) == 0;
     }
     if (any_nl || any_pl) HIPCK(launch_lists(e->p, list_grids(e), (int)k, 0, e->stream));
-    HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), (int)k, e->cur_lds, e->stream));
+    HIPCK(launch_step(e->p, MODE_STEP, lpb, step_grids(e), (int)k, e->stream));
   }
   HIPCK(launch_advance(e->p, (int)nsteps, e->stream));
   return SOP_OK;
@@ -872,6 +911,10 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
   if (step_lds_limit(e->lds_limit) != hipSuccess) {
     (void)hipGetLastError();
     e->lds_limit = 65536 - 2048;
+  }
+  {
+    const int64_t whole = (int64_t)(e->lds_limit / 16) / TILE * TILE;
+    e->lds_big_cap = e->opts.lds_beads_max > 0 && whole > e->opts.lds_beads_max ? whole : 0;
   }
   // A blocking stream: synchronous hipMemcpy calls on the null stream order
   // after the engine's kernels, so host reads never race device writes.
@@ -1092,8 +1135,8 @@ sop_status sop_force_eval(sop_engine *e, int32_t sys, int32_t with_noise) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, grid_items(e),
-                    grid_items_far(e), 0, e->cur_lds, e->stream));
+  HIPCK(launch_step(e->p, with_noise ? MODE_FORCE_NOISE : MODE_FORCE, e->opts.lanes_per_bead, step_grids(e), 0,
+                    e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   return engine_check_errors(e, &sys, 1);
 }
@@ -1268,8 +1311,7 @@ sop_status sop_get_wrapped_coords(sop_engine *e, int32_t sys, float *xyz) {
 static sop_status noisy_forces(sop_engine *e, int32_t sys, int kstep, std::vector<float4> &f) {
   sop_status st = engine_set_batch(e, &sys, 1);
   if (st) return st;
-  HIPCK(launch_step(e->p, MODE_FORCE_NOISE, e->opts.lanes_per_bead, grid_items(e), grid_items_far(e), kstep, e->cur_lds,
-                    e->stream));
+  HIPCK(launch_step(e->p, MODE_FORCE_NOISE, e->opts.lanes_per_bead, step_grids(e), kstep, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
   st = engine_check_errors(e, &sys, 1);
   if (st) return st;
@@ -1417,7 +1459,7 @@ sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys,
   HIPCK(hipEventCreate(&b));
   const int lpb = e->opts.lanes_per_bead;
   HIPCK(hipEventRecord(a, e->stream));
-  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, grid_items(e), grid_items_far(e), r, e->cur_lds, e->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(launch_step(e->p, MODE_STEP, lpb, step_grids(e), r, e->stream));
   HIPCK(hipEventRecord(b, e->stream));
   HIPCK(hipEventSynchronize(b));
   float ms = 0.0f;

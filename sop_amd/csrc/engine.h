@@ -19,8 +19,7 @@
 namespace sopk {
 
 hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st);
-hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int grid_far, int kstep, size_t lds,
-                       hipStream_t st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg, int kstep, hipStream_t st);
 hipError_t step_lds_limit(size_t bytes);
 hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st);
 hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st);
@@ -85,9 +84,11 @@ struct sop_engine {
   std::vector<int32_t> h_work, h_syslist;
   std::vector<int4> h_items;
   int64_t work_cap = 0, syslist_cap = 0, items_cap = 0;
-  int32_t cur_nwork = 0, cur_nsys = 0, cur_nitems = 0, cur_nitems_far = 0;
+  int32_t cur_nwork = 0, cur_nsys = 0, cur_nitems = 0, cur_nitems_big = 0, cur_nitems_far = 0;
   size_t cur_lds = 0;     // dynamic LDS of the step kernel for the current batch
+  size_t cur_lds_big = 0; // dynamic LDS of the whole-CU tier
   size_t lds_limit = 0;   // largest dynamic LDS the step kernel may use
+  int64_t lds_big_cap = 0; // largest system (beads) of the whole-CU LDS tier; 0: no such tier
   int32_t step_blocks = 0; // step-kernel grid (1024-thread blocks); 0 = 2 per compute unit
   bool fixed_grids = false; // serving: launch shapes independent of the batch, so
                             // one captured graph serves every batch composition
@@ -118,5 +119,7 @@ int grid_tiles(const sop_engine *e);
 int grid_sys(const sop_engine *e);
 int grid_items(const sop_engine *e);
 int grid_items_far(const sop_engine *e);
+int grid_items_big(const sop_engine *e);
+StepGrids step_grids(const sop_engine *e);
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d);
 } // namespace sopk

@@ -1384,6 +1384,23 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
 }
 
+// Work items of one launch: LDS items (tier 0), whole-CU LDS items (tier 1),
+// or the items that gather from global memory (KLDS = false).
+template <bool KLDS>
+__device__ __forceinline__ void item_range(const DevPtrs &p, int tier, int &w0, int &w1) {
+  const int n0 = *p.nitems, n1 = *p.nitems_big;
+  if (!KLDS) {
+    w0 = n0 + n1;
+    w1 = w0 + *p.nitems_far;
+  } else if (tier == 0) {
+    w0 = 0;
+    w1 = n0;
+  } else {
+    w0 = n0;
+    w1 = n0 + n1;
+  }
+}
+
 // Fused step over work items {first tile, tiles (<= STEP_WAVES), first staged
 // bead or -1, staged beads}: wave w of the block takes tile first + w.  The
 // LDS kernel (KLDS) takes the items whose systems fit in LDS and first copies
@@ -1393,11 +1410,11 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
 // nitems_far)).  The result is identical either way.  Both are launched each
 // step, so each has its own register budget.
 template <int LPB, int MODE, bool KLDS>
-__device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *s_pos) {
+__device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *s_pos, int tier) {
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: system data in scalar registers
-  const int w0 = KLDS ? 0 : *p.nitems;
-  const int w1 = KLDS ? *p.nitems : w0 + *p.nitems_far;
+  int w0, w1;
+  item_range<KLDS>(p, tier, w0, w1);
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1596,12 +1613,12 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
 // Shell filter over the step kernel's work items (same LDS staging): wave w
 // of the block takes tile first + w of its item.
 template <bool KLDS>
-__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep) {
+__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier) {
   extern __shared__ float4 s_pos[];
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int w0 = KLDS ? 0 : *p.nitems;
-  const int w1 = KLDS ? *p.nitems : w0 + *p.nitems_far;
+  int w0, w1;
+  item_range<KLDS>(p, tier, w0, w1);
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1626,14 +1643,14 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
 
 // 8 waves per SIMD: two blocks per compute unit
 template <int LPB, int MODE>
-__global__ __launch_bounds__(STEP_THREADS, 8) void k_step(DevPtrs p, int kstep) {
+__global__ __launch_bounds__(STEP_THREADS, 8) void k_step(DevPtrs p, int kstep, int tier) {
   extern __shared__ float4 s_pos[];
-  step_items<LPB, MODE, true>(p, kstep, s_pos);
+  step_items<LPB, MODE, true>(p, kstep, s_pos, tier);
 }
 
 template <int LPB, int MODE>
 __global__ __launch_bounds__(STEP_THREADS) void k_step_far(DevPtrs p, int kstep) {
-  step_items<LPB, MODE, false>(p, kstep, nullptr);
+  step_items<LPB, MODE, false>(p, kstep, nullptr, 0);
 }
 
 // ------------------------------------------------ energies / observables
@@ -1890,24 +1907,31 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   hipLaunchKernelGGL(k_pl_natives, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
-  if (lg.grid_items > 0)
-    hipLaunchKernelGGL(k_shell_filter<true>, dim3(lg.grid_items), dim3(STEP_THREADS), lg.lds, st, p, kstep);
-  if (lg.grid_far > 0)
-    hipLaunchKernelGGL(k_shell_filter<false>, dim3(lg.grid_far), dim3(STEP_THREADS), 0, st, p, kstep);
+  const StepGrids &sg = lg.sg;
+  if (sg.grid > 0)
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS), sg.lds, st, p, kstep, 0);
+  if (sg.grid_big > 0)
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, st, p, kstep, 1);
+  if (sg.grid_far > 0)
+    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 0, st, p, kstep, 0);
   hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   return SOPK_LAUNCH_CHECK();
 }
 
 template <int MODE>
-static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid, int grid_far, int kstep, size_t lds,
-                                   hipStream_t st) {
-  if (grid > 0) switch (lpb) {
-    case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
-    case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
-    case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
-    default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep); break;
+static hipError_t launch_step_mode(const DevPtrs &p, int lpb, const StepGrids &sg, int kstep, hipStream_t st) {
+  for (int tier = 0; tier < 2; ++tier) {
+    const int grid = tier ? sg.grid_big : sg.grid;
+    const size_t lds = tier ? sg.lds_big : sg.lds;
+    if (grid > 0) switch (lpb) {
+      case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
+      case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
+      case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
+      default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
+    }
   }
+  const int grid_far = sg.grid_far;
   if (grid_far > 0) switch (lpb) {
     case 4: hipLaunchKernelGGL((k_step_far<4, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
     case 8: hipLaunchKernelGGL((k_step_far<8, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
@@ -1917,11 +1941,10 @@ static hipError_t launch_step_mode(const DevPtrs &p, int lpb, int grid, int grid
   return SOPK_LAUNCH_CHECK();
 }
 
-hipError_t launch_step(const DevPtrs &p, int mode, int lpb, int grid, int grid_far, int kstep, size_t lds,
-                       hipStream_t st) {
-  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, grid, grid_far, kstep, lds, st);
-  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, grid, grid_far, kstep, lds, st);
-  return launch_step_mode<MODE_STEP>(p, lpb, grid, grid_far, kstep, lds, st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg, int kstep, hipStream_t st) {
+  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, sg, kstep, st);
+  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, sg, kstep, st);
+  return launch_step_mode<MODE_STEP>(p, lpb, sg, kstep, st);
 }
 
 // Allow the step kernels dynamic LDS up to `bytes`.

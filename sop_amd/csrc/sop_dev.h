@@ -136,7 +136,8 @@ struct DevPtrs {
   int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
   int4 *items;          // step-kernel work: {first tile, tiles, first staged bead or -1, staged beads}
   int32_t *nitems;       // items that stage their systems in LDS: items[0 .. *nitems)
-  int32_t *nitems_far;   // items that gather from global memory: items[*nitems .. + *nitems_far)
+  int32_t *nitems_big;   // items staged in a whole-CU LDS block: items[*nitems .. + *nitems_big)
+  int32_t *nitems_far;   // items that gather from global memory: the *nitems_far after those
 };
 
 // step kernel: blocks of STEP_WAVES waves, one 64-bead tile per wave; an item
@@ -144,11 +145,19 @@ struct DevPtrs {
 constexpr int STEP_WAVES = 16;
 constexpr int STEP_THREADS = 64 * STEP_WAVES;
 
+// Grids of the step kernel's three launches: items staged in LDS at two
+// blocks per compute unit, items staged in a whole compute unit's LDS, items
+// that gather from global memory.
+struct StepGrids {
+  int grid, grid_big, grid_far;
+  size_t lds, lds_big;
+};
+
 // Grids of the list kernels: per tile, per system, and the step kernel's
 // work items (the shell filter shares its LDS staging).
 struct ListGrids {
-  int grid_tiles, grid_sys, grid_items, grid_far;
-  size_t lds;
+  int grid_tiles, grid_sys;
+  StepGrids sg;
 };
 
 // systems whose cell sort fits one block's LDS
