@@ -132,6 +132,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--lds-beads", type=int, default=0, help="LDS-resident step for systems up to this size "
                     "(0 = engine default, < 0 = off)")
+    ap.add_argument("--pl-cap", type=int, default=0, help="per-bead pair-list capacity bound (0: the engine's)")
     ap.add_argument("--pl-shell", type=float, default=None,
                     help="pair-list shell width in A (default: the engine's, 3 A; < 0 = none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -153,7 +154,8 @@ def main():
     beads = sum(s.n for s in systems)
     log(f"workload: {len(systems)} replicas, {beads} beads, generated in {time.perf_counter() - t:.1f}s")
 
-    eng = S.Engine(0, lanes_per_bead=args.lanes, lds_beads_max=args.lds_beads, pl_shell=args.pl_shell)
+    eng = S.Engine(0, lanes_per_bead=args.lanes, lds_beads_max=args.lds_beads, pl_shell=args.pl_shell,
+                   pl_cap_max=args.pl_cap)
     t = time.perf_counter()
     ids = [eng.add(s) for s in systems]
     log(f"resident on device in {time.perf_counter() - t:.1f}s")
