@@ -129,9 +129,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--lds-beads", type=int, default=0, help="LDS-resident step for systems up to this size "
                     "(0 = engine default, < 0 = off)")
+    ap.add_argument("--serve-lanes", type=int, default=0,
+                    help="lanes per bead of the served leg's engine (0: the batched leg's, after autotune)")
     ap.add_argument("--pl-cap", type=int, default=0, help="per-bead pair-list capacity bound (0: the engine's)")
     ap.add_argument("--pl-shell", type=float, default=None,
                     help="pair-list shell width in A (default: the engine's, 3 A; < 0 = none)")
@@ -192,7 +194,7 @@ def main():
     if args.serve_seconds > 0:
         try:
             from sop_amd import serving
-            served = serving.bench_served(args.serve_seconds, log)
+            served = serving.bench_served(args.serve_seconds, log, lanes_per_bead=args.serve_lanes or lanes)
         except (ImportError, NotImplementedError) as ex:
             log(f"served leg skipped: {ex}")
 

@@ -193,7 +193,7 @@ def make_chain_system(seed: int, n: int, params: Optional[Params] = None, native
 class Engine:
     """One HIP device's engine (libsop).  Not thread-safe."""
 
-    def __init__(self, device=0, lanes_per_bead=16, grid_blocks=0, use_graphs=True, nl_cap_max=0,
+    def __init__(self, device=0, lanes_per_bead=4, grid_blocks=0, use_graphs=True, nl_cap_max=0,
                  pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None):
         self._h = None
         o = _abi.EngineOpts()

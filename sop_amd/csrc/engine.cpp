@@ -861,7 +861,7 @@ void sop_engine_opts_default(sop_engine_opts *o) {
   o->nl_cap_max = 4096;
   o->pl_cap_max = 1024;
   o->cell_cap_max = 32768;
-  o->lanes_per_bead = 16;
+  o->lanes_per_bead = 4;
   o->grid_blocks = 0;
   o->use_graphs = 1;
   o->lds_beads_max = 5056;  // two 1024-thread step blocks per compute unit at 80,896 B each
