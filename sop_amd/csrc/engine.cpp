@@ -804,7 +804,7 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
     HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < G; ++k) {
       if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, list_grids(e), k, 0, e->stream);
-      (void)launch_step(e->p, MODE_STEP, lpb, step_grids(e), k, e->stream);
+      (void)launch_step(e->p, MODE_STEP, lpb, step_grids(e), k, e->stream, &e->fork);
     }
     (void)launch_advance(e->p, G, e->stream);
     HIPCK(hipStreamEndCapture(e->stream, &graph));
@@ -837,7 +837,7 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
       any_pl |= ((st[q] + k) % npl[q]) == 0;
     }
     if (any_nl || any_pl) HIPCK(launch_lists(e->p, list_grids(e), (int)k, 0, e->stream));
-    HIPCK(launch_step(e->p, MODE_STEP, lpb, step_grids(e), (int)k, e->stream));
+    HIPCK(launch_step(e->p, MODE_STEP, lpb, step_grids(e), (int)k, e->stream, &e->fork));
   }
   HIPCK(launch_advance(e->p, (int)nsteps, e->stream));
   return SOP_OK;
@@ -922,6 +922,13 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     delete e;
     return SOP_ERR_HIP;
   }
+  if (hipStreamCreate(&e->fork.side) != hipSuccess ||
+      hipEventCreateWithFlags(&e->fork.fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->fork.join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    sop_engine_destroy(e);
+    return SOP_ERR_HIP;
+  }
   bind_arrays(e);
   // Lane-layout self test of the DPP / swizzle / ballot helpers.
   int *d_out = nullptr;
@@ -980,6 +987,10 @@ void sop_engine_destroy(sop_engine *e) {
   void *others[] = {e->p.nlrep_j, e->p.plrep_j, e->p.slrep_j,  e->p.tile_sys, e->p.sys,  e->p.sysout,
                     e->p.syslist, e->p.work,    e->p.nwork,    e->p.cellwork, e->p.items};
   for (void *q : others) if (q) note(hipFree(q), "hipFree");
+  if (e->fork.side) note(hipStreamSynchronize(e->fork.side), "hipStreamSynchronize");
+  if (e->fork.fork) note(hipEventDestroy(e->fork.fork), "hipEventDestroy");
+  if (e->fork.join) note(hipEventDestroy(e->fork.join), "hipEventDestroy");
+  if (e->fork.side) note(hipStreamDestroy(e->fork.side), "hipStreamDestroy");
   if (e->stream) note(hipStreamDestroy(e->stream), "hipStreamDestroy");
   if (first != hipSuccess) std::fprintf(stderr, "sop_engine_destroy: %s: %s\n", what, hipGetErrorString(first));
   (void)hipGetLastError();

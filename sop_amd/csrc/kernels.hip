@@ -1920,7 +1920,17 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
 }
 
 template <int MODE>
-static hipError_t launch_step_mode(const DevPtrs &p, int lpb, const StepGrids &sg, int kstep, hipStream_t st) {
+static hipError_t launch_step_mode(const DevPtrs &p, int lpb, const StepGrids &sg, int kstep, hipStream_t st,
+                                   const StepFork *fk) {
+  // the global-gather launch beside the LDS launches when both have work
+  const bool fork = fk && sg.grid_far > 0 && (sg.grid > 0 || sg.grid_big > 0);
+  hipStream_t far_st = st;
+  if (fork) {
+    hipError_t h = hipEventRecord(fk->fork, st);
+    if (h == hipSuccess) h = hipStreamWaitEvent(fk->side, fk->fork, 0);
+    if (h != hipSuccess) return h;
+    far_st = fk->side;
+  }
   for (int tier = 0; tier < 2; ++tier) {
     const int grid = tier ? sg.grid_big : sg.grid;
     const size_t lds = tier ? sg.lds_big : sg.lds;
@@ -1933,18 +1943,24 @@ static hipError_t launch_step_mode(const DevPtrs &p, int lpb, const StepGrids &s
   }
   const int grid_far = sg.grid_far;
   if (grid_far > 0) switch (lpb) {
-    case 4: hipLaunchKernelGGL((k_step_far<4, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
-    case 8: hipLaunchKernelGGL((k_step_far<8, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
-    case 32: hipLaunchKernelGGL((k_step_far<32, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
-    default: hipLaunchKernelGGL((k_step_far<16, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, st, p, kstep); break;
+    case 4: hipLaunchKernelGGL((k_step_far<4, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, far_st, p, kstep); break;
+    case 8: hipLaunchKernelGGL((k_step_far<8, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, far_st, p, kstep); break;
+    case 32: hipLaunchKernelGGL((k_step_far<32, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, far_st, p, kstep); break;
+    default: hipLaunchKernelGGL((k_step_far<16, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, far_st, p, kstep); break;
   }
-  return SOPK_LAUNCH_CHECK();
+  hipError_t h = SOPK_LAUNCH_CHECK();
+  if (fork && h == hipSuccess) {
+    h = hipEventRecord(fk->join, fk->side);
+    if (h == hipSuccess) h = hipStreamWaitEvent(st, fk->join, 0);
+  }
+  return h;
 }
 
-hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg, int kstep, hipStream_t st) {
-  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, sg, kstep, st);
-  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, sg, kstep, st);
-  return launch_step_mode<MODE_STEP>(p, lpb, sg, kstep, st);
+hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg, int kstep, hipStream_t st,
+                       const StepFork *fk) {
+  if (mode == MODE_FORCE) return launch_step_mode<MODE_FORCE>(p, lpb, sg, kstep, st, fk);
+  if (mode == MODE_FORCE_NOISE) return launch_step_mode<MODE_FORCE_NOISE>(p, lpb, sg, kstep, st, fk);
+  return launch_step_mode<MODE_STEP>(p, lpb, sg, kstep, st, fk);
 }
 
 // Allow the step kernels dynamic LDS up to `bytes`.

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <hip/hip_runtime_api.h>
 
 namespace sopk {
 
@@ -151,6 +152,14 @@ constexpr int STEP_THREADS = 64 * STEP_WAVES;
 struct StepGrids {
   int grid, grid_big, grid_far;
   size_t lds, lds_big;
+};
+
+// A second stream and two events: the step launch that gathers from global
+// memory runs beside the LDS launches (a fork and a join in the captured
+// graph) instead of after them.
+struct StepFork {
+  hipStream_t side;
+  hipEvent_t fork, join;
 };
 
 // Grids of the list kernels: per tile, per system, and the step kernel's
