@@ -803,7 +803,7 @@ sop_status engine_run_block_graph(sop_engine *e, int32_t G) {
     const int lpb = e->opts.lanes_per_bead;
     HIPCK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < G; ++k) {
-      if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, list_grids(e), k, 0, e->stream);
+      if (k % n_pl == 0 || k % n_nl == 0) (void)launch_lists(e->p, list_grids(e), k, 0, e->stream, &e->fork);
       (void)launch_step(e->p, MODE_STEP, lpb, step_grids(e), k, e->stream, &e->fork);
     }
     (void)launch_advance(e->p, G, e->stream);
@@ -836,7 +836,7 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
       any_nl |= ((st[q] + k) % nnl[q]) == 0;
       any_pl |= ((st[q] + k) % npl[q]) == 0;
     }
-    if (any_nl || any_pl) HIPCK(launch_lists(e->p, list_grids(e), (int)k, 0, e->stream));
+    if (any_nl || any_pl) HIPCK(launch_lists(e->p, list_grids(e), (int)k, 0, e->stream, &e->fork));
     HIPCK(launch_step(e->p, MODE_STEP, lpb, step_grids(e), (int)k, e->stream, &e->fork));
   }
   HIPCK(launch_advance(e->p, (int)nsteps, e->stream));

@@ -18,7 +18,8 @@
 
 namespace sopk {
 
-hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st);
+hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st,
+                        const StepFork *fk = nullptr);
 hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg, int kstep, hipStream_t st,
                        const StepFork *fk = nullptr);
 hipError_t step_lds_limit(size_t bytes);
@@ -69,7 +70,7 @@ struct Pools {
 struct sop_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  sopk::StepFork fork{nullptr, nullptr, nullptr};  // global-gather step launches beside the LDS ones
+  sopk::StepFork fork{nullptr, nullptr, nullptr};  // whole-CU and global-gather launches beside the LDS ones
   int num_cu = 0;
   sop_engine_opts opts{};
   std::string err;

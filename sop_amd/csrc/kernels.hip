@@ -1890,10 +1890,31 @@ __global__ void k_selftest(int *out, float *fout) {
 
 #define SOPK_LAUNCH_CHECK() hipGetLastError()
 
+// The whole-CU and global-gather launches of the step kernel and of the shell
+// filter go on a side stream beside the two-blocks-per-unit LDS launch when
+// both sides have work: fork (side waits for st) and join (st waits for
+// side).  Under graph capture these become two graph branches.
+static bool fork_wanted(const StepFork *fk, const StepGrids &sg) {
+  return fk && sg.grid > 0 && (sg.grid_big > 0 || sg.grid_far > 0);
+}
+
+static hipError_t fork_begin(const StepFork *fk, hipStream_t st) {
+  hipError_t h = hipEventRecord(fk->fork, st);
+  if (h == hipSuccess) h = hipStreamWaitEvent(fk->side, fk->fork, 0);
+  return h;
+}
+
+static hipError_t fork_end(const StepFork *fk, hipStream_t st) {
+  hipError_t h = hipEventRecord(fk->join, fk->side);
+  if (h == hipSuccess) h = hipStreamWaitEvent(st, fk->join, 0);
+  return h;
+}
+
 // One list-rebuild point for the current batch: snapshot / decision, cell
 // grid and counting sort, then whichever of the direct pair-list build or the
 // neighbor-list materialization + filter each system needs.
-hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st) {
+hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int flags, hipStream_t st,
+                        const StepFork *fk) {
   const int grid_tiles = lg.grid_tiles, grid_sys = lg.grid_sys;
   hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(SKIN_THREADS), 0, st, p, kstep, flags);
   hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep);
@@ -1908,12 +1929,22 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
   const StepGrids &sg = lg.sg;
+  const bool fork = fork_wanted(fk, sg);
+  if (fork) {
+    const hipError_t h = fork_begin(fk, st);
+    if (h != hipSuccess) return h;
+  }
+  const hipStream_t side = fork ? fk->side : st;
   if (sg.grid > 0)
     hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS), sg.lds, st, p, kstep, 0);
   if (sg.grid_big > 0)
-    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, st, p, kstep, 1);
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, side, p, kstep, 1);
   if (sg.grid_far > 0)
-    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 0, st, p, kstep, 0);
+    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 0, side, p, kstep, 0);
+  if (fork) {
+    const hipError_t h = fork_end(fk, st);
+    if (h != hipSuccess) return h;
+  }
   hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   return SOPK_LAUNCH_CHECK();
@@ -1922,23 +1953,21 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
 template <int MODE>
 static hipError_t launch_step_mode(const DevPtrs &p, int lpb, const StepGrids &sg, int kstep, hipStream_t st,
                                    const StepFork *fk) {
-  // the global-gather launch beside the LDS launches when both have work
-  const bool fork = fk && sg.grid_far > 0 && (sg.grid > 0 || sg.grid_big > 0);
-  hipStream_t far_st = st;
+  const bool fork = fork_wanted(fk, sg);
   if (fork) {
-    hipError_t h = hipEventRecord(fk->fork, st);
-    if (h == hipSuccess) h = hipStreamWaitEvent(fk->side, fk->fork, 0);
+    const hipError_t h = fork_begin(fk, st);
     if (h != hipSuccess) return h;
-    far_st = fk->side;
   }
+  const hipStream_t far_st = fork ? fk->side : st;
   for (int tier = 0; tier < 2; ++tier) {
     const int grid = tier ? sg.grid_big : sg.grid;
     const size_t lds = tier ? sg.lds_big : sg.lds;
+    hipStream_t ts = tier ? far_st : st;
     if (grid > 0) switch (lpb) {
-      case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
-      case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
-      case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
-      default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(STEP_THREADS), lds, st, p, kstep, tier); break;
+      case 4: hipLaunchKernelGGL((k_step<4, MODE>), dim3(grid), dim3(STEP_THREADS), lds, ts, p, kstep, tier); break;
+      case 8: hipLaunchKernelGGL((k_step<8, MODE>), dim3(grid), dim3(STEP_THREADS), lds, ts, p, kstep, tier); break;
+      case 32: hipLaunchKernelGGL((k_step<32, MODE>), dim3(grid), dim3(STEP_THREADS), lds, ts, p, kstep, tier); break;
+      default: hipLaunchKernelGGL((k_step<16, MODE>), dim3(grid), dim3(STEP_THREADS), lds, ts, p, kstep, tier); break;
     }
   }
   const int grid_far = sg.grid_far;
@@ -1949,10 +1978,7 @@ static hipError_t launch_step_mode(const DevPtrs &p, int lpb, const StepGrids &s
     default: hipLaunchKernelGGL((k_step_far<16, MODE>), dim3(grid_far), dim3(STEP_THREADS), 0, far_st, p, kstep); break;
   }
   hipError_t h = SOPK_LAUNCH_CHECK();
-  if (fork && h == hipSuccess) {
-    h = hipEventRecord(fk->join, fk->side);
-    if (h == hipSuccess) h = hipStreamWaitEvent(st, fk->join, 0);
-  }
+  if (fork && h == hipSuccess) h = fork_end(fk, st);
   return h;
 }
 

@@ -154,9 +154,9 @@ struct StepGrids {
   size_t lds, lds_big;
 };
 
-// A second stream and two events: the step launch that gathers from global
-// memory runs beside the LDS launches (a fork and a join in the captured
-// graph) instead of after them.
+// A second stream and two events: the step launches of the whole-CU and
+// global-gather tiers run beside the two-blocks-per-unit LDS launch (a fork
+// and a join in the captured graph) instead of after it.
 struct StepFork {
   hipStream_t side;
   hipEvent_t fork, join;

@@ -3,7 +3,8 @@
 A system is stepped from coordinates staged in LDS at two blocks per compute
 unit (up to 5,056 beads), staged in a whole compute unit's LDS (up to 10,112
 beads), or gathered from global memory (larger, or with LDS staging turned
-off; that launch runs on a second stream beside the LDS launches).  The
+off).  The whole-CU and global-gather launches run on a second stream beside
+the two-blocks-per-unit launch.  The
 arithmetic is the same in every tier, so trajectories and pair lists must be
 bitwise identical.
 """
@@ -16,8 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_tiers_are_bitwise_identical():
-    # one system per tier: LDS, whole-CU LDS, global gathers (the last runs
-    # on the side stream beside the LDS launches)
+    # one system per tier: LDS, whole-CU LDS, global gathers (the last two
+    # run on the side stream beside the first)
     systems = [S.make_chain_system(31, 3000), S.make_chain_system(32, 7000), S.make_chain_system(33, 11000)]
     with S.Engine(0) as a, S.Engine(0, lds_beads_max=-1) as b:
         ia = [a.add(s) for s in systems]
