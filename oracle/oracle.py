@@ -38,6 +38,7 @@ def lib():
         _lib.orc_dist2_f.restype = C.c_float
         _lib.orc_nl.argtypes = [P(OrcSys), P(C.c_float), P(C.c_int32), C.c_int64, P(C.c_int64),
                                 P(C.c_int32), P(C.c_int32), C.c_int64, P(C.c_int64)]
+        _lib.orc_nl_grid.argtypes = _lib.orc_nl.argtypes
         _lib.orc_pl.argtypes = [P(OrcSys), P(C.c_float), P(C.c_int32), C.c_int64, P(C.c_int32), P(C.c_int32),
                                 C.c_int64, P(C.c_int32), P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]
         _lib.orc_forces.argtypes = [P(OrcSys), P(C.c_double), P(C.c_int32), C.c_int64, P(C.c_int32),
@@ -46,6 +47,7 @@ def lib():
                                          P(C.c_int32), P(C.c_double)]
         _lib.orc_run.argtypes = [P(OrcSys), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64, C.c_float,
                                  C.c_float, C.c_int32, C.c_int32, P(C.c_double)]
+        _lib.orc_run_grid.argtypes = _lib.orc_run.argtypes
         _lib.orc_port_create.argtypes = [P(OrcSys)]
         _lib.orc_run_ud.argtypes = [P(OrcSys), P(C.c_float), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64,
                                     P(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
@@ -130,19 +132,26 @@ class Oracle:
         self.s = s
         self.params = p
 
-    def nl(self, xyz):
-        """Brute-force neighbor list: (native indices k, rep pairs (m,2) with i<j)."""
+    def nl(self, xyz, grid=False):
+        """Neighbor list: (native indices k, rep pairs (m,2) with i<j, row-major).
+        grid=False: brute force over all pairs (orc_nl); grid=True: the same
+        arrays from candidate cells (orc_nl_grid, open boundaries only)."""
         x = np.ascontiguousarray(xyz, np.float32).reshape(-1)
         cap_a = max(1, len(self.nat_i))
-        cap_r = max(1, self.n * (self.n - 1) // 2)
-        ak = np.zeros(cap_a, np.int32)
-        ri = np.zeros(cap_r, np.int32)
-        rj = np.zeros(cap_r, np.int32)
-        na, nr = C.c_int64(), C.c_int64()
-        st = lib().orc_nl(C.byref(self.s), _p(x, C.c_float), _p(ak, C.c_int32), cap_a, C.byref(na),
-                          _p(ri, C.c_int32), _p(rj, C.c_int32), cap_r, C.byref(nr))
-        assert st == 0, st
-        return ak[:na.value].copy(), np.stack([ri[:nr.value], rj[:nr.value]], 1)
+        cap_r = max(1, 192 * self.n) if grid else max(1, self.n * (self.n - 1) // 2)
+        fn = lib().orc_nl_grid if grid else lib().orc_nl
+        while True:
+            ak = np.zeros(cap_a, np.int32)
+            ri = np.zeros(cap_r, np.int32)
+            rj = np.zeros(cap_r, np.int32)
+            na, nr = C.c_int64(), C.c_int64()
+            st = fn(C.byref(self.s), _p(x, C.c_float), _p(ak, C.c_int32), cap_a, C.byref(na),
+                    _p(ri, C.c_int32), _p(rj, C.c_int32), cap_r, C.byref(nr))
+            if st == -1 and grid and nr.value > cap_r:
+                cap_r = nr.value
+                continue
+            assert st == 0, st
+            return ak[:na.value].copy(), np.stack([ri[:nr.value], rj[:nr.value]], 1)
 
     def pl(self, xyz, nl):
         x = np.ascontiguousarray(xyz, np.float32).reshape(-1)
@@ -159,8 +168,8 @@ class Oracle:
                      _p(oj, C.c_int32), C.byref(nr))
         return oa[:na.value].copy(), np.stack([oi[:nr.value], oj[:nr.value]], 1)
 
-    def lists(self, xyz):
-        nl = self.nl(xyz)
+    def lists(self, xyz, grid=False):
+        nl = self.nl(xyz, grid)
         return nl, self.pl(xyz, nl)
 
     def forces(self, xyz, pl):
@@ -185,12 +194,14 @@ class Oracle:
                               C.byref(r))
         return q.value, c.value, r.value
 
-    def run(self, x0, step0, nsteps, seed, a5, var, n_nl, n_pl):
-        """FP64 trajectory; returns final coordinates (n,3) float64."""
+    def run(self, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, grid=False):
+        """FP64 trajectory; returns final coordinates (n,3) float64.  grid=True
+        builds the neighbor lists with orc_nl_grid (open boundaries only)."""
         x = np.ascontiguousarray(x0, np.float32).reshape(-1)
         out = np.zeros(3 * self.n, np.float64)
-        st = lib().orc_run(C.byref(self.s), _p(x, C.c_float), step0, nsteps, seed, a5, var, n_nl, n_pl,
-                           _p(out, C.c_double))
+        fn = lib().orc_run_grid if grid else lib().orc_run
+        st = fn(C.byref(self.s), _p(x, C.c_float), step0, nsteps, seed, a5, var, n_nl, n_pl,
+                _p(out, C.c_double))
         assert st == 0, st
         return out.reshape(-1, 3)
 

@@ -203,10 +203,8 @@ static int is_native(const nat_index *ix, int i, int j) {
 /* Neighbor list by brute force over the implicit master list.  att: indices k
  * into the native table; rep: half pairs (i<j) in row-major order. Returns 0,
  * or -1 if a capacity is too small (counts are still reported). */
-ORC_EXPORT int orc_nl(const orc_sys *s, const float *x, int32_t *att_k, int64_t att_cap,
-                      int64_t *n_att, int32_t *rep_i, int32_t *rep_j, int64_t rep_cap,
-                      int64_t *n_rep) {
-  int64_t na = 0, nr = 0;
+static int64_t nl_natives(const orc_sys *s, const float *x, int32_t *att_k, int64_t att_cap) {
+  int64_t na = 0;
   for (int k = 0; k < s->n_nat; ++k) {
     int i = s->nat_i[k], j = s->nat_j[k];
     float rc = orc_cutoff(s->nl_factor, s->nl_abs, s->nat_r0[k]);
@@ -216,6 +214,13 @@ ORC_EXPORT int orc_nl(const orc_sys *s, const float *x, int32_t *att_k, int64_t 
       na++;
     }
   }
+  return na;
+}
+
+ORC_EXPORT int orc_nl(const orc_sys *s, const float *x, int32_t *att_k, int64_t att_cap,
+                      int64_t *n_att, int32_t *rep_i, int32_t *rep_j, int64_t rep_cap,
+                      int64_t *n_rep) {
+  int64_t na = nl_natives(s, x, att_k, att_cap), nr = 0;
   nat_index ix;
   if (nat_index_build(s, &ix)) return -2;
   float rc2t[9];
@@ -234,6 +239,125 @@ ORC_EXPORT int orc_nl(const orc_sys *s, const float *x, int32_t *att_k, int64_t 
     }
   }
   nat_index_free(&ix);
+  *n_att = na;
+  *n_rep = nr;
+  return (na > att_cap || nr > rep_cap) ? -1 : 0;
+}
+
+/* The neighbor list of orc_nl for open boundaries (boxl == 0), with the
+ * candidate pairs taken from a uniform grid instead of all n(n-1)/2 pairs, for
+ * systems too large for the brute-force loop.  The output arrays are identical
+ * to orc_nl's: the same FP32 predicate on the same pairs, rows in increasing
+ * i, partners in increasing j.  The grid drops only pairs in cells that are
+ * not adjacent along some axis.  Cells are computed in double from the
+ * minimum corner with an edge of at least rc_max + 1 A, so such a pair has a
+ * coordinate difference above rc_max in exact arithmetic; the FP32 difference
+ * is then >= rc (rounding is monotone and rc is a float), its square >= rc*rc
+ * as rounded, and adding non-negative terms keeps d2 >= rc2: the predicate
+ * d2 < rc2 rejects it in orc_nl as well.  Returns -4 for a periodic box,
+ * otherwise as orc_nl. */
+ORC_EXPORT int orc_nl_grid(const orc_sys *s, const float *x, int32_t *att_k, int64_t att_cap,
+                           int64_t *n_att, int32_t *rep_i, int32_t *rep_j, int64_t rep_cap,
+                           int64_t *n_rep) {
+  if (s->boxl > 0.0f) return -4;
+  const int n = s->n;
+  int64_t na = nl_natives(s, x, att_k, att_cap), nr = 0;
+  float rc2t[9], rcmax = 0.0f;
+  for (int a = 0; a < 9; ++a) {
+    float rc = orc_cutoff(s->nl_factor, s->nl_abs, s->sigma_rep[a]);
+    rc2t[a] = rc * rc;
+    if (rc > rcmax) rcmax = rc;
+  }
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double v = (double)x[3 * i + d];
+      if (i == 0 || v < lo[d]) lo[d] = v;
+      if (i == 0 || v > hi[d]) hi[d] = v;
+    }
+  double edge = (double)rcmax + 1.0;
+  int64_t dims[3], ncell;
+  for (;;) {  /* wider cells (still valid) when the grid would outnumber the beads */
+    ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+      dims[d] = (int64_t)floor((hi[d] - lo[d]) / edge) + 1;
+      ncell *= dims[d];
+    }
+    if (ncell <= 8 * (int64_t)n + 64) break;
+    edge *= 1.25;
+  }
+  int32_t *cell = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+  int32_t *start = (int32_t *)calloc((size_t)ncell + 1, sizeof(int32_t));
+  int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+  int64_t bcap = 1024;
+  int32_t *buf = (int32_t *)malloc(sizeof(int32_t) * (size_t)bcap);
+  nat_index ix;
+  if (!cell || !start || !order || !buf || nat_index_build(s, &ix)) {
+    free(cell); free(start); free(order); free(buf);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i) {
+    int64_t c = 0;
+    for (int d = 2; d >= 0; --d) {
+      int64_t q = (int64_t)floor(((double)x[3 * i + d] - lo[d]) / edge);
+      if (q < 0) q = 0;
+      if (q >= dims[d]) q = dims[d] - 1;
+      c = c * dims[d] + q;
+    }
+    cell[i] = (int32_t)c;
+    start[c + 1]++;
+  }
+  for (int64_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+  {
+    int32_t *cur = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncell + 1);
+    if (!cur) { free(cell); free(start); free(order); free(buf); nat_index_free(&ix); return -2; }
+    memcpy(cur, start, sizeof(int32_t) * (size_t)ncell);
+    for (int i = 0; i < n; ++i) order[cur[cell[i]]++] = i;
+    free(cur);
+  }
+  int status = 0;
+  for (int i = 0; i < n && !status; ++i) {
+    int64_t c = cell[i], q[3];
+    q[0] = c % dims[0];
+    q[1] = (c / dims[0]) % dims[1];
+    q[2] = c / (dims[0] * dims[1]);
+    int64_t m = 0;
+    for (int64_t z = q[2] - 1; z <= q[2] + 1; ++z) {
+      if (z < 0 || z >= dims[2]) continue;
+      for (int64_t y = q[1] - 1; y <= q[1] + 1; ++y) {
+        if (y < 0 || y >= dims[1]) continue;
+        for (int64_t w = q[0] - 1; w <= q[0] + 1; ++w) {
+          if (w < 0 || w >= dims[0]) continue;
+          int64_t cc = (z * dims[1] + y) * dims[0] + w;
+          for (int32_t t = start[cc]; t < start[cc + 1]; ++t) {
+            int j = order[t];
+            if (j <= i || !(dist2_f(x, i, j, s->boxl) < rc2t[s->type[i] * 3 + s->type[j]])) continue;
+            if (!excluded(s, i, j) && !is_native(&ix, i, j)) {
+              if (m == bcap) {
+                int32_t *nb = (int32_t *)realloc(buf, sizeof(int32_t) * (size_t)(2 * bcap));
+                if (!nb) { status = -2; break; }
+                buf = nb;
+                bcap *= 2;
+              }
+              buf[m++] = j;
+            }
+          }
+          if (status) break;
+        }
+        if (status) break;
+      }
+      if (status) break;
+    }
+    if (status) break;
+    qsort(buf, (size_t)m, sizeof(int32_t), cmp_i32);
+    for (int64_t k = 0; k < m; ++k) {
+      if (nr < rep_cap) { rep_i[nr] = i; rep_j[nr] = buf[k]; }
+      nr++;
+    }
+  }
+  free(cell); free(start); free(order); free(buf);
+  nat_index_free(&ix);
+  if (status) return status;
   *n_att = na;
   *n_rep = nr;
   return (na > att_cap || nr > rep_cap) ? -1 : 0;
@@ -396,9 +520,9 @@ typedef struct {
   int64_t nla, nlr, pla, plr, cap_rep;
 } orc_lists;
 
-static int lists_alloc(const orc_sys *s, orc_lists *L) {
-  int64_t n = s->n;
-  L->cap_rep = n * (n - 1) / 2 + 1;
+/* cap_rep: pair capacity of the repulsive lists (grown by nl_build). */
+static int lists_alloc(const orc_sys *s, orc_lists *L, int64_t cap_rep) {
+  L->cap_rep = cap_rep;
   L->nla_k = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s->n_nat + 1));
   L->pla_k = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s->n_nat + 1));
   L->nlr_i = (int32_t *)malloc(sizeof(int32_t) * (size_t)L->cap_rep);
@@ -413,18 +537,40 @@ static void lists_free(orc_lists *L) {
   free(L->nla_k); free(L->pla_k); free(L->nlr_i); free(L->nlr_j); free(L->plr_i); free(L->plr_j);
 }
 
-/* Runs nsteps from x0 (float) with step counter starting at step0; writes the
- * final FP64 coordinates into x_out (3n).  Lists are built with the FP32
- * predicate on the coordinates rounded to float.  Returns 0 or <0. */
-ORC_EXPORT int orc_run(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps,
-                       uint64_t seed, float a5, float var, int32_t n_nl, int32_t n_pl,
-                       double *x_out) {
+/* Brute-force lists get capacity for every pair up front; grid lists start
+ * at 192 pairs per bead and grow to the count the last build reported. */
+static int64_t lists_cap(const orc_sys *s, int grid) {
+  int64_t n = s->n;
+  return grid ? 192 * n + 1024 : n * (n - 1) / 2 + 1;
+}
+
+static int nl_build(const orc_sys *s, const float *xf, orc_lists *L, int grid) {
+  for (;;) {
+    int st = grid ? orc_nl_grid(s, xf, L->nla_k, s->n_nat + 1, &L->nla, L->nlr_i, L->nlr_j, L->cap_rep, &L->nlr)
+                  : orc_nl(s, xf, L->nla_k, s->n_nat + 1, &L->nla, L->nlr_i, L->nlr_j, L->cap_rep, &L->nlr);
+    if (st != -1 || L->nlr <= L->cap_rep) return st;
+    int64_t cap = L->nlr + L->nlr / 4 + 1024;
+    int32_t *a = (int32_t *)realloc(L->nlr_i, sizeof(int32_t) * (size_t)cap);
+    if (a) L->nlr_i = a;
+    int32_t *b = (int32_t *)realloc(L->nlr_j, sizeof(int32_t) * (size_t)cap);
+    if (b) L->nlr_j = b;
+    int32_t *c = (int32_t *)realloc(L->plr_i, sizeof(int32_t) * (size_t)cap);
+    if (c) L->plr_i = c;
+    int32_t *d = (int32_t *)realloc(L->plr_j, sizeof(int32_t) * (size_t)cap);
+    if (d) L->plr_j = d;
+    if (!a || !b || !c || !d) return -2;
+    L->cap_rep = cap;
+  }
+}
+
+static int run_od(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps, uint64_t seed, float a5,
+                  float var, int32_t n_nl, int32_t n_pl, double *x_out, int grid) {
   int n = s->n;
   double *x = x_out;
   float *xf = (float *)malloc(sizeof(float) * 3 * (size_t)n);
   double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
   orc_lists L;
-  int st = lists_alloc(s, &L);
+  int st = lists_alloc(s, &L, lists_cap(s, grid));
   if (!xf || !f || st) { free(xf); free(f); lists_free(&L); return -2; }
   for (int i = 0; i < 3 * n; ++i) x[i] = (double)x0[i];
   const double A5 = a5, VAR = var;
@@ -432,7 +578,7 @@ ORC_EXPORT int orc_run(const orc_sys *s, const float *x0, int64_t step0, int64_t
     int64_t step = step0 + it;
     for (int i = 0; i < 3 * n; ++i) xf[i] = (float)x[i];
     if (it == 0 || step % n_nl == 0) {
-      st = orc_nl(s, xf, L.nla_k, s->n_nat + 1, &L.nla, L.nlr_i, L.nlr_j, L.cap_rep, &L.nlr);
+      st = nl_build(s, xf, &L, grid);
       if (st) break;
     }
     if (it == 0 || step % n_pl == 0)
@@ -448,6 +594,23 @@ ORC_EXPORT int orc_run(const orc_sys *s, const float *x0, int64_t step0, int64_t
   }
   free(xf); free(f); lists_free(&L);
   return st;
+}
+
+/* Runs nsteps from x0 (float) with step counter starting at step0; writes the
+ * final FP64 coordinates into x_out (3n).  Lists are built with the FP32
+ * predicate on the coordinates rounded to float.  Returns 0 or <0. */
+ORC_EXPORT int orc_run(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps,
+                       uint64_t seed, float a5, float var, int32_t n_nl, int32_t n_pl,
+                       double *x_out) {
+  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 0);
+}
+
+/* orc_run with the neighbor list of orc_nl_grid (open boundaries only). */
+ORC_EXPORT int orc_run_grid(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps,
+                            uint64_t seed, float a5, float var, int32_t n_nl, int32_t n_pl,
+                            double *x_out) {
+  if (s->boxl > 0.0f) return -4;
+  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 1);
 }
 
 /* Underdamped Langevin dynamics (sop.cu:421-484, coefficients
@@ -471,7 +634,7 @@ ORC_EXPORT int orc_run_ud(const orc_sys *s, const float *x0, const float *v0, in
   double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
   double *u = (double *)malloc(sizeof(double) * 3 * (size_t)n);
   orc_lists L;
-  int st = lists_alloc(s, &L);
+  int st = lists_alloc(s, &L, lists_cap(s, 0));
   if (!xf || !f || !u || st) { free(xf); free(f); free(u); lists_free(&L); return -2; }
   for (int i = 0; i < 3 * n; ++i) {
     x[i] = (double)x0[i];

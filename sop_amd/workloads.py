@@ -7,7 +7,7 @@ C3  512 replicas, N log-uniform in [200, 5000], T uniform in [0.5, 0.9],
 C4  served stream: Poisson arrivals, N log-normal (median 1500, sigma 0.9)
     clipped to [100, 20000], steps log-uniform, T uniform in [0.5, 0.9] (seed 4)
 C5  1000 chains x 500 beads on a 10x10x10 lattice at 60 A, intra-chain natives
-    only, no periodic box (seed 5)
+    only, no periodic box, no two beads of different chains within 4 A (seed 5)
 
 Chains are compact self-avoiding walks (3.8 A bonds, no non-bonded bead within
 4 A) confined to the globule volume of 140 A^3 per bead; natives are the pairs
@@ -79,14 +79,53 @@ def c4_system(job, seed=4):
     return s
 
 
-def c5(nchains=1000, per=500, spacing=60.0, seed=5):
+def _rotation(rng):
+    """Uniform random rotation matrix (QR of a Gaussian matrix, signs fixed)."""
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def c5(nchains=1000, per=500, spacing=60.0, seed=5, min_dist=4.0, tries=64):
+    """Chains go onto the lattice in site order.  A chain with a bead closer
+    than min_dist (the walk's own non-bonded minimum) to a bead of a chain
+    already placed on one of the 26 neighbouring sites is turned about its
+    centroid by a seeded random rotation, up to `tries` times, and otherwise
+    regenerated from a new seed; sites two or more apart are >= 120 A away,
+    beyond the reach of two confined 500-bead walks."""
+    from scipy.spatial import cKDTree
+
     g = int(round(nchains ** (1 / 3)))
+    rng = np.random.default_rng(seed)
+    placed = {}
     xs, bs, nis, njs, nrs, ch = [], [], [], [], [], []
     off = 0
     for c in range(nchains):
-        x, b = generate_chain(seed * 1000003 + c, per)
-        ix, iy, iz = c % g, (c // g) % g, c // (g * g)
-        x = x - x.mean(0) + spacing * np.array([ix, iy, iz], np.float32)
+        site = (c % g, (c // g) % g, c // (g * g))
+        near = [placed[(site[0] + a, site[1] + b, site[2] + d)]
+                for a in (-1, 0, 1) for b in (-1, 0, 1) for d in (-1, 0, 1)
+                if (site[0] + a, site[1] + b, site[2] + d) in placed]
+        tree = cKDTree(np.concatenate(near)) if near else None
+        gen_seed = seed * 1000003 + c
+        while True:
+            x0, b = generate_chain(gen_seed, per)
+            x0 = x0.astype(np.float64)
+            x0 -= x0.mean(0)
+            for t in range(tries):
+                y = (x0 if t == 0 else x0 @ _rotation(rng).T) + spacing * np.array(site, np.float64)
+                x = y.astype(np.float32)
+                if tree is None or not np.isfinite(tree.query(x, distance_upper_bound=min_dist)[0]).any():
+                    break
+            else:
+                gen_seed += 7919 * nchains
+                continue
+            break
+        # bond r0 = the bond lengths as placed (rotation moves them by float rounding)
+        b = b.copy()
+        b[:-1] = np.linalg.norm(x[1:].astype(np.float64) - x[:-1], axis=1).astype(np.float32)
+        placed[site] = x
         ni, nj, nr = native_contacts(x, None, 8.0)
         xs.append(x)
         bs.append(b)

@@ -102,6 +102,24 @@ def test_generator_is_seeded_compact_and_self_avoiding(n):
     assert rg < R
 
 
+def test_assembly_chains_keep_their_distance():
+    # the 1,000-chain lattice assembly: no two beads of different chains (nor
+    # non-bonded beads of one chain) within 4 A, bonds at their r0, and
+    # natives only inside chains
+    from scipy.spatial import cKDTree
+    from sop_amd import workloads as W
+    s = W.c5()
+    assert s.n == 500000 and len(np.unique(s.chain)) == 1000
+    pairs = cKDTree(s.xyz).query_pairs(4.0 - 1e-4, output_type="ndarray")
+    bonded = (s.chain[pairs[:, 0]] == s.chain[pairs[:, 1]]) & (np.abs(pairs[:, 0] - pairs[:, 1]) == 1)
+    assert bonded.all(), pairs[~bonded][:5]
+    x = s.xyz.astype(np.float64)
+    same = s.chain[1:] == s.chain[:-1]
+    d = np.linalg.norm(x[1:] - x[:-1], axis=1)
+    assert np.abs(d[same] - s.bond_r0[:-1][same]).max() < 1e-5
+    assert (s.chain[s.nat_i] == s.chain[s.nat_j]).all()
+
+
 def test_native_contacts_match_brute_force():
     x, _ = S.generate_chain(3, 300)
     ch = np.repeat([0, 1, 2], 100).astype(np.int32)

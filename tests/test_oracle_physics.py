@@ -9,7 +9,9 @@ single instead of double rounding of the 3.2 factor) fails it:
   virial     W = -dE(lambda x)/dlambda at lambda = 1 (pair-potential identity)
   lists      constructed geometries with known membership (strict <), the
              double-then-float rounding of 3.2*r0 (sop.cu:746), minimum-image
-             ties (rint half-even)
+             ties (rint half-even); the grid-built neighbor list (for
+             systems too large for the all-pairs loop) equals the all-pairs
+             one array for array
   integrator noise off: one step equals x + a5 F; free beads: <dr^2> = 6 (T/zeta) t
   Rg, Q      pair-sum Rg^2 equals the centroid form; Q at scaled native states
 """
@@ -242,6 +244,57 @@ def test_min_image_ties_round_half_even():
     assert O.min_image_f(49.0, L) == np.float32(49.0)
     assert O.min_image_f(51.0, L) == np.float32(-49.0)
     assert O.min_image_f(51.0, 0.0) == np.float32(51.0)   # no box
+
+
+def grid_cases():
+    rng = np.random.default_rng(17)
+    # a 3,000-bead globule cut into 7 chains, two bead types, factor cutoffs
+    # (3.2 x sigma per type pair) in an open box
+    p = S.params_reference()
+    p.boxl = 0.0
+    yield random_system(41, n=3000, nchains=7, params=p)
+    # the same with the absolute 32 A cutoff
+    yield random_system(42, n=2000, nchains=3)
+    # clusters far apart in a 3 km-wide cloud: the grid widens its cells
+    c = rng.uniform(-1500, 1500, (30, 1, 3))
+    x = (c + rng.normal(0, 8.0, (30, 10, 3))).reshape(-1, 3)
+    yield mk(x, chain=np.arange(300, dtype=np.int32), bond_r0=np.full(300, 3.8, np.float32))
+    # a lattice at 8 A spacing from -100 A: pairs at exactly 32 A (dropped,
+    # strict <) and pairs on both sides of cell faces at exact multiples
+    g = np.arange(7, dtype=np.float32) * 8.0 - 100.0
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    yield mk(x, chain=np.arange(len(x), dtype=np.int32), bond_r0=np.full(len(x), 3.8, np.float32))
+    # a single bead
+    yield mk(np.zeros((1, 3)), bond_r0=np.full(1, 3.8, np.float32))
+
+
+def test_grid_neighbor_list_equals_brute_force():
+    # orc_nl_grid must return orc_nl's arrays exactly: same pairs, same
+    # row-major order (a stencil missing a neighbour cell, a cell computed
+    # from the wrong corner or an unsorted row fails this)
+    for k, s in enumerate(grid_cases()):
+        o = Oracle(s)
+        a = o.nl(s.xyz)
+        b = o.nl(s.xyz, grid=True)
+        assert np.array_equal(a[0], b[0]), k
+        assert np.array_equal(a[1], b[1]), (k, len(a[1]), len(b[1]))
+        assert k == 4 or len(a[1]) > 0, k
+
+
+def test_grid_neighbor_list_refuses_a_periodic_box():
+    s = random_system(3, n=200, params=S.params_reference())
+    assert s.params.boxl > 0
+    with pytest.raises(AssertionError):
+        Oracle(s).nl(s.xyz, grid=True)
+
+
+def test_grid_trajectory_equals_brute_force():
+    s = random_system(44, n=400, nchains=3)
+    a5, var = S.params_derived(s.params)
+    o = Oracle(s)
+    xa = o.run(s.xyz, 0, 120, 5, a5, var, 50, 10)
+    xb = o.run(s.xyz, 0, 120, 5, a5, var, 50, 10, grid=True)
+    assert np.array_equal(xa, xb)
 
 
 # ------------------------------------------------------------ integrator
