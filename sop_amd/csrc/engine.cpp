@@ -490,6 +490,9 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
       S.rep_guard[q] = sg * 1.0e-7f;
       S.rep_m12[q] = -12.0f * ep * P.rep_c12;
       S.rep_m6[q] = -6.0f * ep * P.rep_c6;
+      const double s6q = std::pow((double)S.sig2_rep[q], 3.0);
+      S.rep_a[q] = (float)((double)S.rep_m12[q] * s6q * s6q);
+      S.rep_b[q] = (float)((double)S.rep_m6[q] * s6q);
       const float rn = cutoff_host(P.nl_factor, P.nl_abs, sg);
       const float rp = cutoff_host(P.pl_factor, P.pl_abs, sg);
       S.rc2_nl_rep[q] = rn * rn;

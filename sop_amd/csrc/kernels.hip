@@ -130,9 +130,10 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
 __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
-    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    // one 32x32->64 product per multiplier (v_mad_u64_u32) instead of
+    // separate low and high multiplies
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
@@ -1272,7 +1273,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       }
       // native LJ over the native pair list, the next entry's loads ahead
       {
-        const float eA = S.eps_att[row], eB = ONE ? eA : S.eps_att[row + 1];
+        // c = -12 eps u q6 (q6 - 1) with u = 1/d2, q6 = (r0^2 u)^3, as
+        // fma(t, q6, -t) with t = -12 eps u q6
+        const float eA = -12.0f * S.eps_att[row], eB = ONE ? eA : -12.0f * S.eps_att[row + 1];
         for (int k = l; k < nc; k += LPB) {
           const int j = jn;
           const float r0 = r0n;
@@ -1288,7 +1291,8 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           const float ir2 = rcpf(d2);
           const float q2 = r02 * ir2;
           const float q6 = q2 * q2 * q2;
-          float c = -12.0f * eps * ir2 * q6 * (q6 - 1.0f);
+          const float t = eps * ir2 * q6;
+          float c = fmaf(t, q6, -t);
           c = d2 > 1.0e-7f * r02 ? c : 0.0f;
           fx = fmaf(c, dx, fx);
           fy = fmaf(c, dy, fy);
@@ -1300,19 +1304,18 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       // chunk's gathers; the row's padding entries are the bead itself and
       // contribute nothing (d2 = 0 fails the guard)
       {
-        const float s2A = S.sig2_rep[row], s2B = ONE ? s2A : S.sig2_rep[row + 1];
-        const float m12A = S.rep_m12[row], m12B = ONE ? m12A : S.rep_m12[row + 1];
-        const float m6A = S.rep_m6[row], m6B = ONE ? m6A : S.rep_m6[row + 1];
+        // c = u q6 (m12 q6 + m6) with u = 1/d2, q6 = (sigma^2 u)^3, as
+        // u^4 (a u^3 + b) with a = m12 sigma^12, b = m6 sigma^6
+        const float aA = S.rep_a[row], aB = ONE ? aA : S.rep_a[row + 1];
+        const float bA = S.rep_b[row], bB = ONE ? bA : S.rep_b[row + 1];
         const float gA = S.rep_guard[row], gB = ONE ? gA : S.rep_guard[row + 1];
         auto rep = [&](float4 xj) {
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const bool tb = !ONE && w_type(xj.w) == 2;
-          const float s2 = tb ? s2B : s2A;
-          const float ir2 = rcpf(d2);
-          const float q2 = s2 * ir2;
-          const float q6 = q2 * q2 * q2;
-          float c = ir2 * q6 * fmaf(tb ? m12B : m12A, q6, tb ? m6B : m6A);
+          const float u = rcpf(d2);
+          const float u2 = u * u;
+          float c = (u2 * u2) * fmaf(tb ? aB : aA, u2 * u, tb ? bB : bA);
           c = d2 > (tb ? gB : gA) ? c : 0.0f;
           fx = fmaf(c, dx, fx);
           fy = fmaf(c, dy, fy);

@@ -54,6 +54,7 @@ struct SysDev {
   float ss_e6, ss_f6; // eps_ss*sigma_ss^6 and 6*eps_ss*sigma_ss^6
   float eps_att[4];
   float eps_rep[4], sig2_rep[4], rep_guard[4], rep_m12[4], rep_m6[4];
+  float rep_a[4], rep_b[4]; // rep_m12 * sigma^12 and rep_m6 * sigma^6 (step kernel's form)
   float c12, c6;
   float rc2_nl_rep[4], rc2_pl_rep[4];
   double nl_factor, pl_factor;
