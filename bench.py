@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--lds-beads", type=int, default=0, help="LDS-resident step for systems up to this size "
                     "(0 = engine default, < 0 = off)")
+    ap.add_argument("--serve-block", type=int, default=0,
+                    help="steps per served graph replay (0: the jobs' neighbor-list interval)")
     ap.add_argument("--serve-lanes", type=int, default=0,
                     help="lanes per bead of the served leg's engine (0: the batched leg's, after autotune)")
     ap.add_argument("--pl-cap", type=int, default=0, help="per-bead pair-list capacity bound (0: the engine's)")
@@ -194,7 +196,8 @@ def main():
     if args.serve_seconds > 0:
         try:
             from sop_amd import serving
-            served = serving.bench_served(args.serve_seconds, log, lanes_per_bead=args.serve_lanes or lanes)
+            served = serving.bench_served(args.serve_seconds, log, lanes_per_bead=args.serve_lanes or lanes,
+                                          block_steps=args.serve_block)
         except (ImportError, NotImplementedError) as ex:
             log(f"served leg skipped: {ex}")
 

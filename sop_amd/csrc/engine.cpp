@@ -290,7 +290,9 @@ int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
 }
 
 // Validate and upload one system, without its first lists (engine_init_lists).
-sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, int64_t step_end, int32_t *slot_out) {
+sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, int64_t step_end, int32_t *slot_out,
+                                     const Upload *upl) {
+  hipStream_t up = upl ? upl->stream : nullptr;
   if (!d || d->n_beads < 1 || !d->xyz) return set_error(e, SOP_ERR_INVALID, "system: need n_beads >= 1 and xyz");
   const int32_t n = d->n_beads;
   const sop_params &P = d->params;
@@ -352,20 +354,33 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   h.step = d->step0;
   h.parity = 0;
   h.params = P;
+  // with an upload stream, every upload goes through the pinned staging
+  // buffer (a copy from pageable memory would wait for the engine stream)
+  const size_t bl = (size_t)h.bead_len, nl_b = (size_t)h.nat_len;
+  const size_t stage_need = 16 * 256 + 16 * bl * 3 + 4 * bl * 7 + 8 * nl_b + bl + 4 * (bl / TILE) + sizeof(SysDev);
+  if (up && stage_need > upl->stage_bytes) return SOP_ERR_BUSY;
   sop_status st;
-  h.bead_off = alloc_or_grow(e, e->a_beads, h.bead_len, TILE, 0, &st);
+  // with an upload stream, free ranges only: growing a pool reallocates
+  // arrays that work in flight on the engine stream may be using
+  auto alloc = [&](RangeAlloc &a, int64_t len, int64_t align, int which) -> int64_t {
+    if (!up) return alloc_or_grow(e, a, len, align, which, &st);
+    const int64_t off = a.alloc(len, align);
+    st = off < 0 ? SOP_ERR_BUSY : SOP_OK;
+    return off;
+  };
+  h.bead_off = alloc(e->a_beads, h.bead_len, TILE, 0);
   if (st) return st;
-  h.nat_off = alloc_or_grow(e, e->a_nat, h.nat_len, 1, 1, &st);
+  h.nat_off = alloc(e->a_nat, h.nat_len, 1, 1);
   if (st) { e->a_beads.release(h.bead_off, h.bead_len); return st; }
-  h.nl_off = alloc_or_grow(e, e->a_nl, h.nl_len, 16, 2, &st);
+  h.nl_off = alloc(e->a_nl, h.nl_len, 16, 2);
   if (st) { e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len); return st; }
-  h.pl_off = alloc_or_grow(e, e->a_pl, h.pl_len, 16, 3, &st);
+  h.pl_off = alloc(e->a_pl, h.pl_len, 16, 3);
   if (st) {
     e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
     e->a_nl.release(h.nl_off, h.nl_len);
     return st;
   }
-  h.cell_off = alloc_or_grow(e, e->a_cells, h.cell_len, 1, 4, &st);
+  h.cell_off = alloc(e->a_cells, h.cell_len, 1, 4);
   if (st) {
     e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
     e->a_nl.release(h.nl_off, h.nl_len); e->a_pl.release(h.pl_off, h.pl_len);
@@ -375,6 +390,12 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   int32_t slot = -1;
   for (size_t s = 0; s < e->sys.size(); ++s)
     if (!e->sys[s].used) { slot = (int32_t)s; break; }
+  if (slot < 0 && up) {
+    e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
+    e->a_nl.release(h.nl_off, h.nl_len); e->a_pl.release(h.pl_off, h.pl_len);
+    e->a_cells.release(h.cell_off, h.cell_len);
+    return SOP_ERR_BUSY;
+  }
   if (slot < 0) {
     if (e->fixed_pools && (int64_t)e->sys.size() >= e->cap.slots && e->cap.slots > 0) {
       slot = -1;
@@ -395,6 +416,23 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   std::vector<int32_t> zeros((size_t)h.bead_len, 0);
   std::vector<int32_t> natj((size_t)h.nat_len, 0);
   std::vector<float> natr((size_t)h.nat_len, 0.0f);
+  std::vector<float4> v((size_t)h.bead_len, make_float4(0, 0, 0, 0));
+  std::vector<uint8_t> ord((size_t)h.bead_len);
+  std::vector<int32_t> ts((size_t)(h.bead_len / TILE), slot);
+  SysDev S;
+  // host buffers above stay alive until the uploads that read them are done
+  struct UploadDrain {
+    hipStream_t s;
+    ~UploadDrain() { if (s) (void)hipStreamSynchronize(s); }
+  } drain{up};
+  size_t staged = 0;
+  auto put = [&](void *dst, const void *src, size_t bytes) {
+    if (!up) return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    char *q = static_cast<char *>(upl->stage) + staged;
+    staged += (bytes + 255) / 256 * 256;
+    std::memcpy(q, src, bytes);
+    return hipMemcpyAsync(dst, q, bytes, hipMemcpyHostToDevice, up);
+  };
   h.wbits.resize((size_t)n);
   int32_t cur = 0;
   for (int i = 0; i < n; ++i) {
@@ -413,32 +451,24 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     }
   }
   const DevPtrs &p = e->p;
-  HIPCK(hipMemcpy(p.pos0 + b0, pos.data(), 16 * h.bead_len, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(p.pos1 + b0, pos.data(), 16 * h.bead_len, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(p.bond_r0 + b0, bond.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(p.nat_start + b0, nstart.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(p.nat_cnt + b0, ncnt.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
+  HIPCK(put(p.pos0 + b0, pos.data(), 16 * h.bead_len));
+  HIPCK(put(p.pos1 + b0, pos.data(), 16 * h.bead_len));
+  HIPCK(put(p.bond_r0 + b0, bond.data(), 4 * h.bead_len));
+  HIPCK(put(p.nat_start + b0, nstart.data(), 4 * h.bead_len));
+  HIPCK(put(p.nat_cnt + b0, ncnt.data(), 4 * h.bead_len));
   for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt})
-    HIPCK(hipMemcpy(arr + b0, zeros.data(), 4 * h.bead_len, hipMemcpyHostToDevice));
-  {
-    // underdamped state: the velocities entering the first step (w = 0)
-    std::vector<float4> v((size_t)h.bead_len, make_float4(0, 0, 0, 0));
-    if (d->vel)
-      for (int i = 0; i < n; ++i) v[i] = make_float4(d->vel[3 * i], d->vel[3 * i + 1], d->vel[3 * i + 2], 0.0f);
-    HIPCK(hipMemcpy(p.vel + b0, v.data(), 16 * h.bead_len, hipMemcpyHostToDevice));
-  }
-  HIPCK(hipMemcpy(p.nat_j + h.nat_off, natj.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(p.nat_r0 + h.nat_off, natr.data(), 4 * h.nat_len, hipMemcpyHostToDevice));
-  {
-    // identity bead order per tile until the first pair list orders it
-    std::vector<uint8_t> ord((size_t)h.bead_len);
-    for (size_t k = 0; k < ord.size(); ++k) ord[k] = (uint8_t)(k % TILE);
-    HIPCK(hipMemcpy(p.tile_ord + b0, ord.data(), ord.size(), hipMemcpyHostToDevice));
-  }
-  std::vector<int32_t> ts((size_t)(h.bead_len / TILE), slot);
-  HIPCK(hipMemcpy(p.tile_sys + b0 / TILE, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice));
+    HIPCK(put(arr + b0, zeros.data(), 4 * h.bead_len));
+  // underdamped state: the velocities entering the first step (w = 0)
+  if (d->vel)
+    for (int i = 0; i < n; ++i) v[i] = make_float4(d->vel[3 * i], d->vel[3 * i + 1], d->vel[3 * i + 2], 0.0f);
+  HIPCK(put(p.vel + b0, v.data(), 16 * h.bead_len));
+  HIPCK(put(p.nat_j + h.nat_off, natj.data(), 4 * h.nat_len));
+  HIPCK(put(p.nat_r0 + h.nat_off, natr.data(), 4 * h.nat_len));
+  // identity bead order per tile until the first pair list orders it
+  for (size_t k = 0; k < ord.size(); ++k) ord[k] = (uint8_t)(k % TILE);
+  HIPCK(put(p.tile_ord + b0, ord.data(), ord.size()));
+  HIPCK(put(p.tile_sys + b0 / TILE, ts.data(), 4 * ts.size()));
   // device descriptor
-  SysDev S;
   std::memset(&S, 0, sizeof(S));
   S.bead_off = (int32_t)b0;
   S.n = n;
@@ -544,7 +574,8 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   S.dims[0] = S.dims[1] = S.dims[2] = 1;
   S.ncells = 1;
   S.wrap_free = 0;
-  HIPCK(hipMemcpy(p.sys + slot, &S, sizeof(SysDev), hipMemcpyHostToDevice));
+  HIPCK(put(p.sys + slot, &S, sizeof(SysDev)));
+  if (up) HIPCK(hipStreamSynchronize(up));
   e->sys[slot] = std::move(h);
   *slot_out = slot;
   return SOP_OK;

@@ -110,7 +110,20 @@ namespace sopk {
 sop_status set_error(sop_engine *e, sop_status st, const std::string &msg);
 sop_status hip_check(sop_engine *e, hipError_t h, const char *what);
 sop_status engine_add_system(sop_engine *e, const sop_system_desc *d, int32_t *slot);
-sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, int64_t step_end, int32_t *slot);
+// A stream for uploads beside the engine stream's work, and a pinned host
+// buffer every upload is staged through.
+struct Upload {
+  hipStream_t stream;
+  void *stage;
+  size_t stage_bytes;
+};
+// With `upl`, the system goes into free pool ranges and a free slot only
+// (SOP_ERR_BUSY, nothing changed, when it does not fit without growing a
+// pool, or its uploads do not fit the staging buffer) and is uploaded on
+// upl->stream, which the call drains before returning; work in flight on the
+// engine stream is not waited for and not touched.
+sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, int64_t step_end, int32_t *slot,
+                                     const Upload *upl = nullptr);
 sop_status engine_init_lists(sop_engine *e, const int32_t *slots, int32_t n, sop_status *status);
 sop_status engine_remove_system(sop_engine *e, int32_t slot);
 sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots);

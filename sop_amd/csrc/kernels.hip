@@ -1518,7 +1518,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     int32_t *dst32 = p.plrep_j + pl_base + row;
     // chunks in batches of CB per lane, all loads of a batch issued before
     // its stores (a load issued after a store waits for the store as well)
-    constexpr int CB = 6;
+    constexpr int CB = 8;
     const int clast = max(nch - 1, 0);
     int cnt = 0;
     for (int m0 = 0; __ballot(l + 4 * m0 < nch) != 0; m0 += CB) {

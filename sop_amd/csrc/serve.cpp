@@ -49,6 +49,7 @@ struct sop_engine::Serve {
   std::string fatal;
   sop_serve_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Upload up{nullptr, nullptr, 0};  // uploads of admitted jobs beside the block in flight
 };
 
 namespace {
@@ -112,104 +113,138 @@ sop_status retire(sop_engine *e, sop_engine::Serve *sv) {
   return SOP_OK;
 }
 
+// Admission: queued jobs in arrival order while the running-job and HBM
+// budget limits allow.  With `up`, the uploads overlap the block in flight on
+// the engine stream; a job that would need a pool to grow goes back to the
+// queue front with the ones behind it, and *deferred is set so the caller
+// admits them once the engine is idle.
+void admit(sop_engine *e, sop_engine::Serve *sv, const Upload *up, std::vector<sop_engine::Job *> &added,
+           std::vector<int32_t> &added_slots, bool *deferred) {
+  std::vector<sop_engine::Job *> take;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    while (!sv->queue.empty()) {
+      sop_engine::Job *j = sv->queue.front();
+      const size_t held = sv->running.size() + added.size() + take.size();
+      if ((int)held >= sv->max_jobs) break;
+      if (sv->used + j->bytes > sv->budget && held > 0) break;
+      sv->queue.pop_front();
+      sv->used += j->bytes;
+      take.push_back(j);
+    }
+  }
+  for (size_t q = 0; q < take.size(); ++q) {
+    sop_engine::Job *j = take[q];
+    int32_t slot = -1;
+    const sop_status st = engine_add_system_nolists(e, &j->desc, j->desc.step0 + j->nsteps, &slot, up);
+    if (st == SOP_ERR_BUSY && up) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      for (size_t r = take.size(); r-- > q;) {
+        sv->queue.push_front(take[r]);
+        sv->used -= take[r]->bytes;
+      }
+      *deferred = true;
+      return;
+    }
+    if (st != SOP_OK) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      j->t_admit = sop_clock_seconds();
+      j->status = st;
+      j->done = 1;
+      j->t_done = j->t_admit;
+      sv->used -= j->bytes;
+      continue;
+    }
+    j->slot = slot;
+    added.push_back(j);
+    added_slots.push_back(slot);
+  }
+}
+
+// First lists of the admitted jobs (one launch sequence), then into the
+// running batch.
+sop_status start_jobs(sop_engine *e, sop_engine::Serve *sv, const std::vector<sop_engine::Job *> &added,
+                      const std::vector<int32_t> &added_slots) {
+  if (added.empty()) return SOP_OK;
+  std::vector<sop_status> ok(added.size(), SOP_OK);
+  const sop_status st = engine_init_lists(e, added_slots.data(), (int32_t)added.size(), ok.data());
+  if (st != SOP_OK) return st;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  const double now = sop_clock_seconds();
+  for (size_t q = 0; q < added.size(); ++q) {
+    sop_engine::Job *j = added[q];
+    j->t_admit = now;
+    if (ok[q] != SOP_OK) {
+      j->status = ok[q];
+      j->done = 1;
+      j->t_done = now;
+      j->slot = -1;
+      sv->used -= j->bytes;
+      continue;
+    }
+    sv->running.push_back(j);
+  }
+  return SOP_OK;
+}
+
 void serve_loop(sop_engine *e) {
   sop_engine::Serve *sv = e->serve;
   (void)hipSetDevice(e->device);
   for (;;) {
-    // 1. admission
     const double t0 = sop_clock_seconds();
-    std::vector<sop_engine::Job *> admit;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       if (sv->running.empty() && sv->queue.empty() && !sv->stop)
         sv->cv.wait_for(lk, std::chrono::milliseconds(5));
       if (sv->stop) break;
-      while (!sv->queue.empty()) {
-        sop_engine::Job *j = sv->queue.front();
-        if ((int)(sv->running.size() + admit.size()) >= sv->max_jobs) break;
-        if (sv->used + j->bytes > sv->budget && !(sv->running.empty() && admit.empty())) break;
-        sv->queue.pop_front();
-        sv->used += j->bytes;
-        admit.push_back(j);
-      }
     }
-    // upload every admitted job (each runs from its step0 to step0 + nsteps),
-    // then build all their first lists with one launch sequence
+    // 1. one block of steps over the running batch, in flight on the engine stream
+    sop_status st = SOP_OK;
+    const bool run = !sv->running.empty();
+    double t2 = t0, t3 = t0;
+    if (run) {
+      std::vector<int32_t> slots;
+      for (sop_engine::Job *j : sv->running) slots.push_back(j->slot);
+      st = engine_set_batch(e, slots.data(), (int32_t)slots.size());
+      t2 = sop_clock_seconds();
+      if (st == SOP_OK && sv->ev0) (void)hipEventRecord(sv->ev0, e->stream);
+      if (st == SOP_OK) st = engine_run_block_graph(e, sv->block);
+      if (st == SOP_OK && sv->ev1) (void)hipEventRecord(sv->ev1, e->stream);
+      t3 = sop_clock_seconds();
+    }
+    // 2. admission, uploading beside the block (free pool ranges only)
     std::vector<sop_engine::Job *> added;
     std::vector<int32_t> added_slots;
-    for (sop_engine::Job *j : admit) {
-      int32_t slot = -1;
-      const sop_status st = engine_add_system_nolists(e, &j->desc, j->desc.step0 + j->nsteps, &slot);
-      if (st != SOP_OK) {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        j->t_admit = sop_clock_seconds();
-        j->status = st;
-        j->done = 1;
-        j->t_done = j->t_admit;
-        sv->used -= j->bytes;
-        continue;
-      }
-      j->slot = slot;
-      added.push_back(j);
-      added_slots.push_back(slot);
-    }
-    if (!added.empty()) {
-      std::vector<sop_status> ok(added.size(), SOP_OK);
-      if (engine_init_lists(e, added_slots.data(), (int32_t)added.size(), ok.data()) != SOP_OK) {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->fatal = e->err;
-        return;
-      }
-      std::lock_guard<std::mutex> lk(sv->mu);
-      const double now = sop_clock_seconds();
-      for (size_t q = 0; q < added.size(); ++q) {
-        sop_engine::Job *j = added[q];
-        j->t_admit = now;
-        if (ok[q] != SOP_OK) {
-          j->status = ok[q];
-          j->done = 1;
-          j->t_done = now;
-          j->slot = -1;
-          sv->used -= j->bytes;
-          continue;
-        }
-        sv->running.push_back(j);
-      }
-    }
-    const double t1 = sop_clock_seconds();
-    if (sv->running.empty()) {
-      std::lock_guard<std::mutex> lk(sv->mu);
-      sv->stats.host_admit_s += t1 - t0;
-      continue;
-    }
-    // 2. one block of steps over the running batch
-    std::vector<int32_t> slots;
-    for (sop_engine::Job *j : sv->running) slots.push_back(j->slot);
-    sop_status st = engine_set_batch(e, slots.data(), (int32_t)slots.size());
-    const double t2 = sop_clock_seconds();
-    if (st == SOP_OK && sv->ev0) (void)hipEventRecord(sv->ev0, e->stream);
-    if (st == SOP_OK) st = engine_run_block_graph(e, sv->block);
-    if (st == SOP_OK && sv->ev1) (void)hipEventRecord(sv->ev1, e->stream);
-    const double t3 = sop_clock_seconds();
-    // 3. retire finished jobs
+    bool deferred = false;
+    if (st == SOP_OK) admit(e, sv, run && sv->up.stream ? &sv->up : nullptr, added, added_slots, &deferred);
+    const double ta = sop_clock_seconds();
+    // 3. retire finished jobs (waits for the block)
     const size_t before = sv->running.size();
-    if (st == SOP_OK) st = retire(e, sv);
+    if (st == SOP_OK && run) st = retire(e, sv);
+    const size_t retired = before - sv->running.size();
     const double t4 = sop_clock_seconds();
+    // 4. jobs that needed a pool to grow, now that the engine is idle; then the
+    // first lists of every admitted job
+    if (st == SOP_OK && deferred) admit(e, sv, nullptr, added, added_slots, &deferred);
+    if (st == SOP_OK) st = start_jobs(e, sv, added, added_slots);
+    const double t5 = sop_clock_seconds();
     if (st == SOP_OK) {
       float ms = 0.0f;
-      if (sv->ev0 && hipEventElapsedTime(&ms, sv->ev0, sv->ev1) != hipSuccess) ms = 0.0f;
+      if (run && sv->ev0 && hipEventElapsedTime(&ms, sv->ev0, sv->ev1) != hipSuccess) ms = 0.0f;
       std::lock_guard<std::mutex> lk(sv->mu);
       sop_serve_stats &c = sv->stats;
-      c.mean_running = (c.mean_running * (double)c.iterations + (double)before) / (double)(c.iterations + 1);
-      c.iterations += 1;
-      c.jobs_admitted += (int64_t)admit.size();
-      c.jobs_retired += (int64_t)(before - sv->running.size());
-      c.steps_run += sv->block;
-      c.gpu_block_s += 1e-3 * ms;
-      c.host_admit_s += t1 - t0;
-      c.host_batch_s += t2 - t1;
-      c.host_block_s += t3 - t2;
-      c.host_retire_s += t4 - t3;
+      c.host_admit_s += (ta - t3) + (t5 - t4);
+      c.jobs_admitted += (int64_t)added.size();
+      if (run) {
+        c.mean_running = (c.mean_running * (double)c.iterations + (double)before) / (double)(c.iterations + 1);
+        c.iterations += 1;
+        c.jobs_retired += (int64_t)retired;
+        c.steps_run += sv->block;
+        c.gpu_block_s += 1e-3 * ms;
+        c.host_batch_s += t2 - t0;
+        c.host_block_s += t3 - t2;
+        c.host_retire_s += t4 - ta;
+      }
     }
     if (st != SOP_OK) {
       std::lock_guard<std::mutex> lk(sv->mu);
@@ -239,6 +274,14 @@ sop_status sop_serve_start(sop_engine *e, const sop_serve_opts *o) {
   e->serve = sv;
   e->fixed_grids = true;
   if (hipEventCreate(&sv->ev0) != hipSuccess || hipEventCreate(&sv->ev1) != hipSuccess) sv->ev0 = sv->ev1 = nullptr;
+  // uploads beside the running block: a non-blocking stream and 64 MiB of
+  // pinned staging (jobs whose uploads exceed it are admitted between blocks)
+  sv->up.stage_bytes = (size_t)64 << 20;
+  if (hipHostMalloc(&sv->up.stage, sv->up.stage_bytes, hipHostMallocDefault) != hipSuccess) {
+    sv->up.stage = nullptr;
+  } else if (hipStreamCreateWithFlags(&sv->up.stream, hipStreamNonBlocking) != hipSuccess) {
+    sv->up.stream = nullptr;
+  }
   sv->th = std::thread(serve_loop, e);
   return SOP_OK;
 }
@@ -346,6 +389,8 @@ sop_status sop_serve_stop(sop_engine *e) {
   const bool ok = sv->fatal.empty();
   if (sv->ev0) (void)hipEventDestroy(sv->ev0);
   if (sv->ev1) (void)hipEventDestroy(sv->ev1);
+  if (sv->up.stream) (void)hipStreamDestroy(sv->up.stream);
+  if (sv->up.stage) (void)hipHostFree(sv->up.stage);
   delete sv;
   e->serve = nullptr;
   e->fixed_grids = false;

@@ -66,21 +66,22 @@ class Server:
 
 
 def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates=256, seed=4,
-                 budget_gib=32.0, drain_timeout=120.0, lanes_per_bead=4):
+                 budget_gib=32.0, drain_timeout=120.0, lanes_per_bead=4, block_steps=0):
     """Served workload: Poisson arrivals at `rate` jobs/s for `seconds`; job
     sizes log-normal (median 1500, sigma 0.9) clipped to [100, 20000]; steps
     log-uniform in [steps_lo, steps_hi]; T uniform in [0.5, 0.9]; fresh Philox
     key per job.  Chains come from `templates` pre-generated compact walks of
     the drawn sizes (job k uses template k % templates).  Returns latency
     percentiles and aggregate bead-steps/s of the completed jobs.  The
-    engine's step kernel uses `lanes_per_bead` lanes per bead."""
+    engine's step kernel uses `lanes_per_bead` lanes per bead; the server
+    runs `block_steps` steps per graph replay (0: the jobs' n_nl)."""
     jobs = W.c4_jobs(rate=rate, duration=seconds, seed=seed, steps_lo=steps_lo, steps_hi=steps_hi)
     t0 = time.perf_counter()
     base = [W.c4_system(jobs[k], seed) for k in range(min(templates, len(jobs)))]
     log(f"served: {len(jobs)} arrivals over {seconds}s, {len(base)} chain templates in "
         f"{time.perf_counter() - t0:.1f}s")
     eng = Engine(0, lanes_per_bead=lanes_per_bead)
-    srv = Server(eng, hbm_budget_bytes=int(budget_gib * (1 << 30)))
+    srv = Server(eng, hbm_budget_bytes=int(budget_gib * (1 << 30)), block_steps=block_steps)
     ids = []
     start = time.perf_counter()
     clock0 = lib().sop_clock_seconds()
@@ -118,7 +119,7 @@ def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates
     lat = np.array(lat)
     offered = sum(n * nst for (_, n, nst) in ids) / seconds
     out = {"jobs": len(ids), "completed": int(len(lat)), "arrival_rate": rate, "arrival_seconds": seconds,
-           "steps_range": [steps_lo, steps_hi], "templates": len(base),
+           "steps_range": [steps_lo, steps_hi], "templates": len(base), "block_steps": block_steps,
            "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
            "bead_steps_per_s": done_beadsteps / wall if wall > 0 else None,
