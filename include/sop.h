@@ -270,11 +270,13 @@ typedef struct sop_serve_stats {
   int64_t jobs_admitted, jobs_retired;
   int64_t steps_run;      /* sum over iterations of the block length */
   double gpu_block_s;     /* device time of the step blocks (HIP events) */
-  double host_admit_s;    /* uploads and first lists of admitted jobs */
+  double host_admit_s;    /* uploads (beside the block in flight) and first lists of admitted jobs */
   double host_batch_s;    /* work tables of the running batch */
   double host_block_s;    /* wall time around the step blocks */
   double host_retire_s;   /* energies, copies back, frees */
   double mean_running;    /* running jobs per iteration, averaged */
+  int64_t jobs_uploaded_beside; /* admissions uploaded while a block ran */
+  int64_t jobs_deferred;  /* admissions put off until the engine was idle (a pool had to grow) */
 } sop_serve_stats;
 sop_status sop_serve_stats_get(sop_engine *e, sop_serve_stats *out);
 double sop_clock_seconds(void);

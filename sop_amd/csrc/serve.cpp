@@ -116,10 +116,10 @@ sop_status retire(sop_engine *e, sop_engine::Serve *sv) {
 // Admission: queued jobs in arrival order while the running-job and HBM
 // budget limits allow.  With `up`, the uploads overlap the block in flight on
 // the engine stream; a job that would need a pool to grow goes back to the
-// queue front with the ones behind it, and *deferred is set so the caller
+// queue front with the ones behind it, *deferred counts them, and the caller
 // admits them once the engine is idle.
 void admit(sop_engine *e, sop_engine::Serve *sv, const Upload *up, std::vector<sop_engine::Job *> &added,
-           std::vector<int32_t> &added_slots, bool *deferred) {
+           std::vector<int32_t> &added_slots, size_t *deferred) {
   std::vector<sop_engine::Job *> take;
   {
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -143,7 +143,7 @@ void admit(sop_engine *e, sop_engine::Serve *sv, const Upload *up, std::vector<s
         sv->queue.push_front(take[r]);
         sv->used -= take[r]->bytes;
       }
-      *deferred = true;
+      *deferred += take.size() - q;
       return;
     }
     if (st != SOP_OK) {
@@ -215,8 +215,9 @@ void serve_loop(sop_engine *e) {
     // 2. admission, uploading beside the block (free pool ranges only)
     std::vector<sop_engine::Job *> added;
     std::vector<int32_t> added_slots;
-    bool deferred = false;
+    size_t deferred = 0;
     if (st == SOP_OK) admit(e, sv, run && sv->up.stream ? &sv->up : nullptr, added, added_slots, &deferred);
+    const size_t beside = run && sv->up.stream ? added.size() : 0;
     const double ta = sop_clock_seconds();
     // 3. retire finished jobs (waits for the block)
     const size_t before = sv->running.size();
@@ -225,6 +226,7 @@ void serve_loop(sop_engine *e) {
     const double t4 = sop_clock_seconds();
     // 4. jobs that needed a pool to grow, now that the engine is idle; then the
     // first lists of every admitted job
+    const size_t n_deferred = deferred;
     if (st == SOP_OK && deferred) admit(e, sv, nullptr, added, added_slots, &deferred);
     if (st == SOP_OK) st = start_jobs(e, sv, added, added_slots);
     const double t5 = sop_clock_seconds();
@@ -235,6 +237,8 @@ void serve_loop(sop_engine *e) {
       sop_serve_stats &c = sv->stats;
       c.host_admit_s += (ta - t3) + (t5 - t4);
       c.jobs_admitted += (int64_t)added.size();
+      c.jobs_uploaded_beside += (int64_t)beside;
+      c.jobs_deferred += (int64_t)n_deferred;
       if (run) {
         c.mean_running = (c.mean_running * (double)c.iterations + (double)before) / (double)(c.iterations + 1);
         c.iterations += 1;

@@ -2,7 +2,8 @@
 
 - every job's final coordinates and energies are identical (bitwise) to the
   same job run alone on the plain engine API;
-- jobs submitted while others run are admitted and overlap with them;
+- jobs submitted while others run are admitted and overlap with them (their
+  uploads beside the block in flight, or after it when a pool must grow);
 - the job state resident at any time stays within the HBM budget;
 - jobs that break the shared schedule are refused at submission.
 """
@@ -48,7 +49,12 @@ def test_served_results_equal_alone_and_overlap():
         for s, steps in jobs[5:]:
             ids.append(srv.submit(s, steps))
         res = [srv.wait(j) for j in ids]
+        st = srv.stats()
         srv.stop()
+    # the late jobs were uploaded beside a running block, or put off until the
+    # engine was idle because a pool had to grow (both paths give the same results)
+    assert st["jobs_admitted"] == len(jobs)
+    assert st["jobs_uploaded_beside"] + st["jobs_deferred"] > 0, st
     for (s, steps), (r, x) in zip(jobs, res):
         assert r["status"] == 0
         xr, er = alone(s, steps)
@@ -57,6 +63,37 @@ def test_served_results_equal_alone_and_overlap():
     # continuous batching: some job was admitted while another was resident
     iv = sorted((r["t_admit"], r["t_done"]) for r, _ in res)
     assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
+
+
+def test_admission_beside_a_block_and_after_pool_growth():
+    # A (short) and B (long) size the pools; once A has retired, C fits A's
+    # freed ranges and uploads beside B's running block; D is larger than
+    # any free range, needs a pool to grow, and waits for the idle engine
+    a, b = S.make_chain_system(611, 3000), S.make_chain_system(612, 64)
+    c, d = S.make_chain_system(613, 2000), S.make_chain_system(614, 8000)
+    systems, steps = (a, b, c, d), (100, 6000, 100, 100)
+    for k, s in enumerate(systems):
+        s.params.seed = W.philox_key(10, k)
+    with S.Engine(0) as eng:
+        srv = serving.Server(eng)
+        ids = [srv.submit(a, steps[0]), srv.submit(b, steps[1])]
+        srv.wait(ids[0], with_coords=False)
+        ids.append(srv.submit(c, steps[2]))
+        while srv.stats()["jobs_admitted"] < 3:
+            time.sleep(0.001)
+        ids.append(srv.submit(d, steps[3]))
+        res = [srv.wait(j) for j in ids]
+        st = srv.stats()
+        srv.stop()
+    print("admission paths:", st, flush=True)
+    assert all(r["status"] == 0 for r, _ in res)
+    assert res[1][0]["t_done"] > res[3][0]["t_admit"]  # D was admitted while B ran
+    assert st["jobs_uploaded_beside"] >= 1, st
+    assert st["jobs_deferred"] >= 1, st
+    for s, n, (r, x) in zip(systems, steps, res):
+        xr, er = alone(s, n)
+        assert np.array_equal(x, xr), s.n
+        assert np.array_equal(r["energies"], er[:6])
 
 
 def test_admission_respects_hbm_budget():
