@@ -642,12 +642,14 @@ __global__ __launch_bounds__(256) void k_cellwork(DevPtrs p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
   int carry = 0;
+  bool nl = false;  // some system materializes its neighbor list at this rebuild
   for (int q0 = 0; q0 < nsys; q0 += 256) {
     const int q = q0 + tid;
     int v = 0;
     if (q < nsys) {
       const SysDev &S = p.sys[p.syslist[q]];
       v = S.build == BUILD_DIRECT ? S.ncells_used : 0;
+      nl |= S.build == BUILD_NL || S.build == BUILD_NL_ONLY;
     }
     const int incl = wave_incl_scan(v);
     if (lane == 63) wsum[wave] = incl;
@@ -662,9 +664,11 @@ __global__ __launch_bounds__(256) void k_cellwork(DevPtrs p) {
     carry += tot;
     __syncthreads();
   }
+  const int any_nl = __syncthreads_or(nl);
   if (tid == 0) {
     p.cellwork[nsys] = carry;
     *p.ncellwork = carry;
+    *p.nlbuild = any_nl;
   }
 }
 
@@ -1020,6 +1024,7 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
 
 // Wave per bead: materialize the neighbor list of the snapshot pos_ref.
 __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
+  if (*p.nlbuild == 0) return;  // no system of the batch needs it (k_cellwork)
   __shared__ uint32_t s_bits[4][NATBITS_WORDS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int k = lane; k < NATBITS_WORDS; k += 64) s_bits[wave][k] = 0u;
@@ -1074,6 +1079,7 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
 
 // Wave per bead: pair list = materialized neighbor list filtered at x.
 __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
+  if (*p.nlbuild == 0) return;  // no system of the batch needs it (k_cellwork)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
