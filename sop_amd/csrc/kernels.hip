@@ -737,6 +737,12 @@ constexpr int EXCL_WORDS = CAND_MAX / 32;
 // test: a self entry has d2 = 0, which the repulsion guard turns into zero
 // force.  pl_cap is a multiple of 16, so the padding stays inside the row.
 constexpr int ROW_PAD = 4;
+// LDS of one workgroup on gfx950, and the shell filter's row staging: per
+// wave, 16 beads of a round x FILTER_SROW 16-bit entries (longer rows store
+// the rest directly)
+constexpr int LDS_BYTES_MAX = 160 * 1024;
+constexpr int FILTER_SROW = 128;
+constexpr int FILTER_STAGE_BYTES = STEP_WAVES * 16 * FILTER_SROW * 2;
 __device__ __forceinline__ void pad_row(const DevPtrs &p, const SysDev &S, int i, int cnt, int k) {
   const int e = cnt + k;
   if (k < ((ROW_PAD - (cnt & (ROW_PAD - 1))) & (ROW_PAD - 1)) && e < S.pl_cap) {
@@ -1467,9 +1473,10 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
 // chunks l, l + 4, ... of the row, kept entries are compacted per chunk
 // position, so a row comes out in (chunk pass, position, lane) order; rows are
 // padded for the step kernel.
-template <bool LDS, bool IDX16>
+template <bool LDS, bool IDX16, bool STAGE>
 __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, int s, int t,
-                                            const float4 *__restrict__ pin, const float4 *s_pos, int lds0) {
+                                            const float4 *__restrict__ pin, const float4 *s_pos, int lds0,
+                                            uint16_t *s_rows) {
   using chunk_t = typename std::conditional<IDX16, uint2, uint4>::type;
   const int lane = threadIdx.x & 63, g = lane >> 2, l = lane & 3;
   const uint32_t below = (1u << l) - 1u;
@@ -1522,6 +1529,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
                                : reinterpret_cast<const chunk_t *>(p.slrep_j + pl_base + row);
     uint16_t *dst16 = (uint16_t *)(p.plrep_j + pl_base) + row;
     int32_t *dst32 = p.plrep_j + pl_base + row;
+    uint16_t *s_row = STAGE ? s_rows + g * FILTER_SROW : nullptr;  // this group's staged row
     // chunks in batches of CB per lane, all loads of a batch issued before
     // its stores (a load issued after a store waits for the store as well)
     constexpr int CB = 8;
@@ -1559,7 +1567,8 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
           const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
           if (keep) {
             const int o = cnt + __popc(bits & below);
-            if (o < pl_cap) {
+            if (STAGE && o < FILTER_SROW) s_row[o] = (uint16_t)(jj[e] - bo);
+            else if (o < pl_cap) {
               if (IDX16) dst16[o] = (uint16_t)(jj[e] - bo);
               else dst32[o] = jj[e];
             }
@@ -1612,9 +1621,27 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
       }
       // pad to a multiple of ROW_PAD with the bead itself (pad_row)
       if (l < ((ROW_PAD - (c & (ROW_PAD - 1))) & (ROW_PAD - 1)) && c + l < pl_cap) {
-        if (IDX16) dst16[c + l] = (uint16_t)(i - bo);
+        if (STAGE && c + l < FILTER_SROW) s_row[c + l] = (uint16_t)(i - bo);
+        else if (IDX16) dst16[c + l] = (uint16_t)(i - bo);
         else dst32[c + l] = i;
       }
+    }
+    if (STAGE) {
+      // the staged part of each row to global memory in 8-byte chunks (the
+      // row start is 32-byte aligned: pl_cap and pl_base are multiples of 16)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (act) {
+        const int c = min(cnt, pl_cap);
+        const int nst = min((c + ROW_PAD - 1) & ~(ROW_PAD - 1), FILTER_SROW) >> 2;
+        const uint2 *srow2 = reinterpret_cast<const uint2 *>(s_row);
+        uint2 *drow2 = reinterpret_cast<uint2 *>(dst16);
+        for (int k = l; k < nst; k += 4) drow2[k] = srow2[k];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
 }
@@ -1622,10 +1649,16 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
 // Shell filter over the step kernel's work items (same LDS staging): wave w
 // of the block takes tile first + w of its item.
 template <bool KLDS>
-__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier) {
+__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier, int stage_off) {
   extern __shared__ float4 s_pos[];
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // stage_off >= 0: rows of 16-bit entries are compacted in LDS at this byte
+  // offset (FILTER_SROW entries per bead of a round, per wave) and written out
+  // in 8-byte chunks; < 0: kept entries are stored one by one
+  uint16_t *s_rows = stage_off >= 0 ? reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(s_pos) + stage_off) +
+                                          wave * (64 / 4) * FILTER_SROW
+                                    : nullptr;
   int w0, w1;
   item_range<KLDS>(p, tier, w0, w1);
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
@@ -1645,8 +1678,10 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
     if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
-    if (S.idx16) filter_tile<KLDS, true>(p, S, s, t, buf_in(p, S.parity + kstep), s_pos, KLDS ? it.z : 0);
-    else filter_tile<KLDS, false>(p, S, s, t, buf_in(p, S.parity + kstep), s_pos, KLDS ? it.z : 0);
+    const float4 *pin = buf_in(p, S.parity + kstep);
+    if (!S.idx16) filter_tile<KLDS, false, false>(p, S, s, t, pin, s_pos, KLDS ? it.z : 0, nullptr);
+    else if (s_rows) filter_tile<KLDS, true, true>(p, S, s, t, pin, s_pos, KLDS ? it.z : 0, s_rows);
+    else filter_tile<KLDS, true, false>(p, S, s, t, pin, s_pos, KLDS ? it.z : 0, nullptr);
   }
 }
 
@@ -1944,12 +1979,17 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
     if (h != hipSuccess) return h;
   }
   const hipStream_t side = fork ? fk->side : st;
+  // row staging after the staged positions where LDS has room for it
+  const bool stage0 = sg.lds + FILTER_STAGE_BYTES <= LDS_BYTES_MAX;
   if (sg.grid > 0)
-    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS), sg.lds, st, p, kstep, 0);
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS),
+                       sg.lds + (stage0 ? FILTER_STAGE_BYTES : 0), st, p, kstep, 0, stage0 ? (int)sg.lds : -1);
   if (sg.grid_big > 0)
-    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, side, p, kstep, 1);
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, side, p, kstep, 1,
+                       -1);
   if (sg.grid_far > 0)
-    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 0, side, p, kstep, 0);
+    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), FILTER_STAGE_BYTES, side, p,
+                       kstep, 0, 0);
   if (fork) {
     const hipError_t h = fork_end(fk, st);
     if (h != hipSuccess) return h;
@@ -2011,7 +2051,10 @@ hipError_t step_lds_limit(size_t bytes) {
 #undef SOPK_LDS_ATTR
   if (h == hipSuccess)
     h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<true>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES_MAX);
+  if (h == hipSuccess)
+    h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)FILTER_STAGE_BYTES);
   return h;
 }
 
