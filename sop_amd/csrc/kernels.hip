@@ -736,7 +736,7 @@ constexpr int EXCL_WORDS = CAND_MAX / 32;
 // multiple of ROW_PAD, so the step kernel reads whole chunks without a count
 // test: a self entry has d2 = 0, which the repulsion guard turns into zero
 // force.  pl_cap is a multiple of 16, so the padding stays inside the row.
-constexpr int ROW_PAD = 4;
+constexpr int ROW_PAD = 8;
 // LDS of one workgroup on gfx950, and the shell filter's row staging: per
 // wave, 16 beads of a round x FILTER_SROW 16-bit entries (longer rows store
 // the rest directly)
@@ -1228,9 +1228,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     t_nc = p.platt_cnt[b0 + lane];
     t_nr = p.plrep_cnt[b0 + lane];
   }
-  // the non-native rows in chunks of E = ROW_PAD entries (8 or 16 bytes)
-  constexpr int E = ROW_PAD;
-  using chunk_t = typename std::conditional<IDX16, uint2, uint4>::type;
+  // the non-native rows in 16-byte chunks: E = 8 16-bit or 4 32-bit entries
+  constexpr int E = IDX16 ? 8 : 4;
+  using chunk_t = uint4;
   float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;  // force on bead b0 + lane
   for (int r = 0; r < LPB; ++r) {
     const int li = __shfl(ordm, r * GPW + g);
@@ -1336,9 +1336,10 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
         for (int c = l; c < nch; c += LPB) {
           const chunk_t cur = q;
           if (c + LPB < nch) q = rowc[c + LPB];
-          uint32_t ww[4];
+          uint32_t ww[E];
           if constexpr (IDX16) {
             ww[0] = cur.x & 0xFFFFu; ww[1] = cur.x >> 16; ww[2] = cur.y & 0xFFFFu; ww[3] = cur.y >> 16;
+            ww[4 % E] = cur.z & 0xFFFFu; ww[5 % E] = cur.z >> 16; ww[6 % E] = cur.w & 0xFFFFu; ww[7 % E] = cur.w >> 16;
           } else {
             ww[0] = cur.x; ww[1] = cur.y; ww[2] = cur.z; ww[3] = cur.w;
           }
@@ -1620,10 +1621,11 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
         if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
       }
       // pad to a multiple of ROW_PAD with the bead itself (pad_row)
-      if (l < ((ROW_PAD - (c & (ROW_PAD - 1))) & (ROW_PAD - 1)) && c + l < pl_cap) {
-        if (STAGE && c + l < FILTER_SROW) s_row[c + l] = (uint16_t)(i - bo);
-        else if (IDX16) dst16[c + l] = (uint16_t)(i - bo);
-        else dst32[c + l] = i;
+      for (int k = l; k < ((ROW_PAD - (c & (ROW_PAD - 1))) & (ROW_PAD - 1)); k += 4) {
+        if (c + k >= pl_cap) break;
+        if (STAGE && c + k < FILTER_SROW) s_row[c + k] = (uint16_t)(i - bo);
+        else if (IDX16) dst16[c + k] = (uint16_t)(i - bo);
+        else dst32[c + k] = i;
       }
     }
     if (STAGE) {
