@@ -763,6 +763,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->p.nsys = e->p.nwork + 16;
     e->p.ncellwork = e->p.nwork + 32;
     e->p.nlbuild = e->p.nwork + 40;
+    e->p.gsort = e->p.nwork + 41;
     e->p.nitems = e->p.nwork + 48;
     e->p.nitems_far = e->p.nwork + 56;
     e->p.nitems_big = e->p.nwork + 60;

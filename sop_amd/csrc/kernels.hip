@@ -213,6 +213,7 @@ __global__ __launch_bounds__(SKIN_THREADS) void k_skin(DevPtrs p, int kstep, int
   __shared__ int s_build;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
+  if (blockIdx.x == 0 && tid == 0) *p.gsort = 0;  // k_bbox raises it (first kernel of a rebuild)
   for (int q = blockIdx.x; q < nsys; q += gridDim.x) {
     const int s = p.syslist[q];
     SysDev &S = p.sys[s];
@@ -349,6 +350,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
       }
       S.ncells = nc;
       S.sort_lds = S.n <= SORT_N_MAX && nc <= SORT_C_MAX;
+      if (!S.sort_lds) atomicOr(p.gsort, 1);
       if (S.boxl > 0.0f) {
         const int ok = ext_max < 0.5f * S.boxl - S.wrap_margin;
         S.wrap_free = ok;
@@ -370,6 +372,7 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
 
 // Per bead: cell id and histogram.
 __global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep) {
+  if (*p.gsort == 0) return;  // every grid build of the batch sorts in LDS (k_bbox)
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
@@ -387,6 +390,7 @@ __global__ __launch_bounds__(64) void k_cell_count(DevPtrs p, int kstep) {
 
 // Per system: exclusive scan of the histogram (DPP wave scan + wave totals).
 __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
+  if (*p.gsort == 0) return;  // every grid build of the batch sorts in LDS (k_bbox)
   __shared__ int wsum[4], wused[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsys = *p.nsys;
@@ -428,6 +432,7 @@ __global__ __launch_bounds__(256) void k_cell_scan(DevPtrs p) {
 
 // Per bead: scatter into its cell's segment (arbitrary in-cell order).
 __global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p) {
+  if (*p.gsort == 0) return;  // every grid build of the batch sorts in LDS (k_bbox)
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
@@ -448,6 +453,7 @@ __global__ __launch_bounds__(64) void k_cell_scatter(DevPtrs p) {
 // a cell-ordered copy of the coordinates.  The in-cell order, and with it every
 // list, is independent of the order of the atomics above.
 __global__ __launch_bounds__(64) void k_cell_rank(DevPtrs p, int kstep) {
+  if (*p.gsort == 0) return;  // every grid build of the batch sorts in LDS (k_bbox)
   const int nwork = *p.nwork;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const int t = p.work[w];
