@@ -143,7 +143,7 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.cell_pos, 4},   {(void **)&p.nat_sorted, 8},
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
       {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16}};
-  e->nat_arrays = {{(void **)&p.nat_j, 4},   {(void **)&p.nat_r0, 4},   {(void **)&p.nlatt_j, 4}, {(void **)&p.natslot, 8},
+  e->nat_arrays = {{(void **)&p.nat_j, 4},    {(void **)&p.nat_r0, 4},  {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
                     {(void **)&p.cellitems, 4}};

@@ -703,32 +703,6 @@ __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
   }
 }
 
-// Per native CSR entry of a system that builds its shell rows from the cell
-// grid: the partner's cell and sorted position, which k_pl_cells turns into
-// candidate slots.  16 lanes per bead.
-__global__ __launch_bounds__(256) void k_pl_natives(DevPtrs p, int kstep) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gq = lane >> 4, gl = lane & 15;
-  const int nwork = *p.nwork;
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const int t = p.work[w];
-    if (t < 0) continue;
-    const int s = p.tile_sys[t];
-    const SysDev &S = p.sys[s];
-    if (S.build != BUILD_DIRECT) continue;
-    const int bend = min(t * TILE + TILE, S.bead_off + S.n);
-    for (int q = 0; q < TILE / 16; ++q) {
-      const int i = t * TILE + q * 16 + wave * 4 + gq;
-      if (i >= bend) continue;
-      const int ns = p.nat_start[i], nn = p.nat_cnt[i];
-      for (int k = gl; k < nn; k += 16) {
-        const int j = p.nat_j[ns + k];
-        p.natslot[ns + k] = make_int2(p.cell_of[j], p.cell_pos[j]);
-      }
-    }
-  }
-}
-
 // Block per home cell: the shell rows of the cell's beads straight from the
 // shell-cutoff grid.  The bounds of the 9 neighbor rows and all their
 // candidates are staged in LDS together (CAND_MAX at a time); every home bead
@@ -958,9 +932,9 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           for (int k = lane; k < nn + 5; k += 64) {
             int cn, pos, jn;
             if (k < nn) {
-              const int2 sl = p.natslot[ns + k];
-              cn = sl.x;
-              pos = sl.y;
+              const int jp = p.nat_j[ns + k];  // native partner: its cell and place in the cell order
+              cn = p.cell_of[jp];
+              pos = p.cell_pos[jp];
               jn = -1;
             } else {
               jn = ji + (k - nn - 2);
@@ -1977,7 +1951,6 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
   hipLaunchKernelGGL(k_cellwork, dim3(1), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
-  hipLaunchKernelGGL(k_pl_natives, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
   const StepGrids &sg = lg.sg;

@@ -121,7 +121,6 @@ struct DevPtrs {
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
   int32_t *cell_pos;    // per bead: its position in the system's cell-sorted order
   int2 *nat_sorted;     // per sorted position: {nat_start, nat_cnt} of the bead there
-  int2 *natslot;        // per native CSR entry: {cell, sorted position} of the partner
   float4 *vel;          // underdamped state per bead: w = 1: xyz = last increment;
                         // w = 0: xyz = velocity entering the next step
   float4 *fout;         // forces (force-eval modes)
