@@ -1138,20 +1138,22 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
   }
 }
 
-// Per tile of a system whose pair list is rebuilt: the order in which the
-// shell filter's and the step kernel's lane groups take the tile's beads, by
+// Per tile of a system whose pair list is rebuilt from its neighbor list: the
+// order in which the step kernel's lane groups take the tile's beads, by
 // decreasing row length (non-native + native), ties by index, so the beads
 // that one round of lane groups computes together run loops of similar
-// length.  Results do not depend on the order: each bead is computed by one
-// group the same way.
+// length (grid and shell builds: the shell filter computes the same order).
+// Results do not depend on the order: each bead is computed by one group the
+// same way.
 __global__ __launch_bounds__(256) void k_tile_order(DevPtrs p) {
+  if (*p.nlbuild == 0) return;  // shell and grid builds: the shell filter orders their tiles
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwork = *p.nwork;
   for (int w = blockIdx.x * 4 + wave; w < nwork; w += gridDim.x * 4) {
     const int t = p.work[w];
     if (t < 0) continue;
     const SysDev &S = p.sys[p.tile_sys[t]];
-    if (S.build != BUILD_DIRECT && S.build != BUILD_NL && S.build != BUILD_SHELL) continue;
+    if (S.build != BUILD_NL) continue;
     const int b0 = t * TILE, i = b0 + lane;
     // grid and shell builds: shell-row counts (the pair lists are filtered
     // after this); neighbor-list builds: the previous pair-list counts
@@ -1471,10 +1473,23 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   const float rc11 = S.rc2_pl_rep[0], rc12 = S.rc2_pl_rep[1], rc21 = S.rc2_pl_rep[2], rc22 = S.rc2_pl_rep[3];
   // row lengths of the whole tile in one load, picked up by shuffles; beads
   // in the tile's order (k_tile_order), slot q = 16 r + g
-  const int ordm = p.tile_ord[t * TILE + lane];
   const bool tin = t * TILE + lane < bend;
   const int t_nr = tin ? p.slrep_cnt[t * TILE + lane] : 0;
   const int t_ns = tin ? p.nat_start[t * TILE + lane] : 0, t_nn = tin ? p.nat_cnt[t * TILE + lane] : 0;
+  // the tile's bead order for this filter and the step kernel (k_tile_order's
+  // key: shell-row length + the native pair-list length of the last rebuild,
+  // decreasing, ties by index), read before this pass rewrites platt_cnt
+  int ordm;
+  {
+    const int key = tin ? t_nr + p.platt_cnt[t * TILE + lane] : -1;
+    int rank = 0;
+    for (int j = 0; j < TILE; ++j) {
+      const int kj = __shfl(key, j);
+      rank += (kj > key) | ((kj == key) & (j < lane));
+    }
+    p.tile_ord[t * TILE + rank] = (uint8_t)lane;
+    ordm = __builtin_amdgcn_ds_permute(rank << 2, lane);  // slot lane holds bead ordm
+  }
   const double plf = S.pl_factor;
   const float pla = S.pl_abs;
   for (int r = 0; r < 4; ++r) {
