@@ -732,58 +732,6 @@ __device__ __forceinline__ void pad_row(const DevPtrs &p, const SysDev &S, int i
   }
 }
 
-// One home bead against the nc staged candidates [b0, b0 + nc), two per lane
-// per pass: the form for home cells with more candidates than the exclusion
-// bitmaps hold, so bounds, topology and a search of the sorted native partners
-// are tested per candidate.  Appends the partners within the pair cutoff to
-// out in candidate order; returns the new count.
-template <bool PBC>
-__device__ __forceinline__ int pl_scan_home(const float4 *s_cand, const int *s_cj, int nc, float4 xi, int ji,
-                                            int chi, float rcA, float rcB, float boxl, const int32_t *nat_g, int nn,
-                                            int32_t *out, uint16_t *out16, int bo, int cnt, int pl_cap, int lane) {
-  for (int k0 = 0; k0 < nc; k0 += 128) {
-    int jj[2];
-    float4 xj[2];
-    bool pr[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int ci = min(k0 + u * 64 + lane, nc - 1);
-      jj[u] = s_cj[ci];
-      xj[u] = s_cand[ci];
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int j = jj[u];
-      const int wj = __float_as_int(xj[u].w);
-      float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
-      if (PBC) {
-        dx = dx - boxl * rintf(dx / boxl);
-        dy = dy - boxl * rintf(dy / boxl);
-        dz = dz - boxl * rintf(dz / boxl);
-      }
-      const float d2 = (dx * dx + dy * dy) + dz * dz;
-      pr[u] = (d2 < ((wj & 3) == 2 ? rcB : rcA)) & (k0 + u * 64 + lane < nc) & (j != ji) &
-              !(((wj >> 2) == chi) & ((unsigned)(j - ji + 2) <= 4u));
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      if (pr[u] && native_pair(nat_g, nn, jj[u])) pr[u] = false;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const uint64_t mk = __ballot(pr[u]);
-      if (pr[u]) {
-        const int o = cnt + lanes_below(mk);
-        if (o < pl_cap) {
-          if (out16) out16[o] = (uint16_t)(jj[u] - bo);
-          else out[o] = jj[u];
-        }
-      }
-      cnt += __popcll(mk);
-    }
-  }
-  return cnt;
-}
-
 // Output row of one bead as a raw buffer of pl_cap entries: stores past the
 // row's end are dropped by the buffer bounds check (the count still grows,
 // which the caller reports as an overflow).
@@ -807,8 +755,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(void *row, int bytes)
 template <bool ONE, bool IDX16>
 __device__ __forceinline__ int pl_scan_fast(const float4 *s_cand, const int *s_cj, int ncp, float4 xi, float rcA,
                                             float rcB, const uint32_t *excl, __amdgpu_buffer_rsrc_t row, int bo,
-                                            int lane) {
-  int cnt = 0;
+                                            int lane, int cnt) {
   const uint32_t lbit = 1u << (lane & 31);
   for (int k0 = 0; k0 < ncp; k0 += 128) {
     int jj[2];
@@ -855,7 +802,6 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
     const int c = item & 0xFFFFF;
     const SysDev &S = p.sys[s];
     const int bo = S.bead_off, cb = S.cell_base, n = S.n;
-    const float boxl = S.boxl;
     const int pl_cap = S.pl_cap;
     const int64_t pl_base = S.pl_base;
     const float rc2_11 = S.rc2_sl_rep[0], rc2_12 = S.rc2_sl_rep[1], rc2_21 = S.rc2_sl_rep[2],
@@ -888,11 +834,11 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
       __syncthreads();  // everyone has read the bounds before the next cell rewrites them
       continue;
     }
-    const bool exact = T > CAND_MAX;
-    // stage the candidates [b0, b0 + nc); the fast form pads to 128 with
+    const bool exact = T > CAND_MAX;  // candidates staged in passes of CAND_MAX
+    // stage the candidates [b0, b0 + nc), padded to a multiple of 128 with
     // coordinates no pair can reach
     auto stage = [&](int b0, int nc) {
-      const int ncp = exact ? nc : (nc + 127) & ~127;
+      const int ncp = (nc + 127) & ~127;
       for (int ci = tid; ci < ncp; ci += 256) {
         const int idx = b0 + ci;
         if (ci < nc) {
@@ -918,14 +864,21 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         s_hcnt[tid] = 0;
         s_hns[tid] = ni.x;
         s_hnn[tid] = ni.y;
-#pragma unroll
-        for (int w = 0; w < EXCL_WORDS; ++w) s_excl[tid][w] = 0u;
       }
-      __syncthreads();
-      if (!exact) {
-        // exclusion bitmaps: the slot of each native partner and of the bead
-        // itself and its chain neighbours i +- 1, 2, from their cells in this
-        // rebuild's grid (the row) and sorted positions (the offset)
+      // candidate passes of CAND_MAX staged slots (one pass when they all fit,
+      // staged above for every home pass)
+      for (int b0 = 0; b0 < T; b0 += CAND_MAX) {
+        const int nc = min(CAND_MAX, T - b0);
+        if (exact) stage(b0, nc);
+        if (tid < nh) {
+#pragma unroll
+          for (int w = 0; w < EXCL_WORDS; ++w) s_excl[tid][w] = 0u;
+        }
+        __syncthreads();
+        // exclusion bitmaps over this pass's slots: the slot of each native
+        // partner and of the bead itself and its chain neighbours i +- 1, 2,
+        // from their cells in this rebuild's grid (the row) and sorted
+        // positions (the offset)
         for (int h = wave; h < nh; h += 4) {
           const int ji = s_hj[h], ns = s_hns[h], nn = s_hnn[h];
           const int chi = __float_as_int(s_home[h].w) >> 2;
@@ -947,8 +900,10 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
             const int r = (nz - cz + 1) * 3 + (ny - cy + 1);
             const int slot = s_roff[r] + (pos - s_rst[r]);
             if (slot < s_roff[r] || slot >= s_roff[r + 1]) continue;
-            if (jn >= 0 && (s_cj[slot] != jn || (__float_as_int(s_cand[slot].w) >> 2) != chi)) continue;
-            atomicOr(&s_excl[h][slot >> 5], 1u << (slot & 31));
+            const int ls = slot - b0;  // staged position in this pass
+            if (ls < 0 || ls >= nc) continue;
+            if (jn >= 0 && (s_cj[ls] != jn || (__float_as_int(s_cand[ls].w) >> 2) != chi)) continue;
+            atomicOr(&s_excl[h][ls >> 5], 1u << (ls & 31));
           }
         }
         __syncthreads();
@@ -959,40 +914,20 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
           int32_t *out = p.slrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
           uint16_t *out16 = S.idx16 ? (uint16_t *)(p.slrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
-          const int ncp = (T + 127) & ~127;
+          const int ncp = (nc + 127) & ~127;
           const bool one = S.type_mask != 3;
           const float rc1 = S.type_mask == 2 ? rcB : rcA;
           void *row = S.idx16 ? (void *)out16 : (void *)out;
           const __amdgpu_buffer_rsrc_t rr = row_rsrc(row, pl_cap * (S.idx16 ? 2 : 4));
+          const int c0 = s_hcnt[h];
           int cnt;
-          if (one) cnt = S.idx16 ? pl_scan_fast<true, true>(s_cand, s_cj, ncp, xi, rc1, rc1, s_excl[h], rr, bo, lane)
-                                 : pl_scan_fast<true, false>(s_cand, s_cj, ncp, xi, rc1, rc1, s_excl[h], rr, bo, lane);
-          else cnt = S.idx16 ? pl_scan_fast<false, true>(s_cand, s_cj, ncp, xi, rcA, rcB, s_excl[h], rr, bo, lane)
-                             : pl_scan_fast<false, false>(s_cand, s_cj, ncp, xi, rcA, rcB, s_excl[h], rr, bo, lane);
+          if (one) cnt = S.idx16 ? pl_scan_fast<true, true>(s_cand, s_cj, ncp, xi, rc1, rc1, s_excl[h], rr, bo, lane, c0)
+                                 : pl_scan_fast<true, false>(s_cand, s_cj, ncp, xi, rc1, rc1, s_excl[h], rr, bo, lane, c0);
+          else cnt = S.idx16 ? pl_scan_fast<false, true>(s_cand, s_cj, ncp, xi, rcA, rcB, s_excl[h], rr, bo, lane, c0)
+                             : pl_scan_fast<false, false>(s_cand, s_cj, ncp, xi, rcA, rcB, s_excl[h], rr, bo, lane, c0);
           if (lane == 0) s_hcnt[h] = cnt;
         }
-      } else {
-        for (int b0 = 0; b0 < T; b0 += CAND_MAX) {
-          const int nc = min(CAND_MAX, T - b0);
-          stage(b0, nc);
-          __syncthreads();
-          for (int h = wave; h < nh; h += 4) {
-            const float4 xi = s_home[h];
-            const int ji = s_hj[h];
-            const int wi = __float_as_int(xi.w), chi = wi >> 2;
-            const float rcA = (wi & 3) == 2 ? rc2_21 : rc2_11, rcB = (wi & 3) == 2 ? rc2_22 : rc2_12;
-            int32_t *out = p.slrep_j + pl_base + (int64_t)(ji - bo) * pl_cap;
-            uint16_t *out16 = S.idx16 ? (uint16_t *)(p.slrep_j + pl_base) + (int64_t)(ji - bo) * pl_cap : nullptr;
-            const int32_t *natg = p.nat_j + s_hns[h];
-            const int cnt = boxl > 0.0f
-                ? pl_scan_home<true>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, natg, s_hnn[h], out, out16, bo,
-                                     s_hcnt[h], pl_cap, lane)
-                : pl_scan_home<false>(s_cand, s_cj, nc, xi, ji, chi, rcA, rcB, boxl, natg, s_hnn[h], out, out16, bo,
-                                      s_hcnt[h], pl_cap, lane);
-            if (lane == 0) s_hcnt[h] = cnt;
-          }
-          __syncthreads();
-        }
+        __syncthreads();  // the next pass restages and clears the bitmaps
       }
       __syncthreads();
       if (tid < nh) {
