@@ -666,6 +666,14 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   std::vector<int32_t> by_arena(slots, slots + nslots);
   std::sort(by_arena.begin(), by_arena.end(),
             [&](int32_t a2, int32_t b2) { return e->sys[a2].bead_off < e->sys[b2].bead_off; });
+  // small batches: two waves per tile, each taking half of the tile's rounds,
+  // when twice the tiles still fit the wave slots of the LDS launch (two
+  // blocks of STEP_WAVES per compute unit); the step then takes half the
+  // latency of one tile instead of leaving waves idle
+  int64_t total_tiles = 0;
+  for (int32_t q = 0; q < nslots; ++q) total_tiles += e->sys[slots[q]].bead_len / TILE;
+  e->tile_split = total_tiles <= (int64_t)e->num_cu * STEP_WAVES;
+  const int32_t per = e->tile_split ? STEP_WAVES / 2 : STEP_WAVES;  // tiles per item
   std::vector<int4> lds_items, big_items;
   std::vector<std::vector<int4>> far_items;  // per non-LDS system
   size_t lds_big = 0;
@@ -680,13 +688,13 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     const int32_t t0 = (int32_t)(h.bead_off / TILE), nt = (int32_t)(h.bead_len / TILE);
     if (h.bead_len <= lds_cap) {
       lds = std::max(lds, (size_t)h.bead_len * 16);
-      if (nt > STEP_WAVES) {
+      if (nt > per) {
         flush();
-        for (int32_t k = 0; k < nt; k += STEP_WAVES)
-          lds_items.push_back(make_int4(t0 + k, std::min(STEP_WAVES, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
+        for (int32_t k = 0; k < nt; k += per)
+          lds_items.push_back(make_int4(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
         continue;
       }
-      if (cur.x >= 0 && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= STEP_WAVES &&
+      if (cur.x >= 0 && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= per &&
           (int64_t)cur.w + h.bead_len <= lds_cap) {
         cur.y += nt;
         cur.w += (int32_t)h.bead_len;
@@ -698,12 +706,12 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     } else if (h.bead_len <= big_cap) {
       flush();
       lds_big = std::max(lds_big, (size_t)h.bead_len * 16);
-      for (int32_t k = 0; k < nt; k += STEP_WAVES)
-        big_items.push_back(make_int4(t0 + k, std::min(STEP_WAVES, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
+      for (int32_t k = 0; k < nt; k += per)
+        big_items.push_back(make_int4(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
     } else {
       flush();
       std::vector<int4> v;
-      for (int32_t k = 0; k < nt; k += STEP_WAVES) v.push_back(make_int4(t0 + k, std::min(STEP_WAVES, nt - k), -1, 0));
+      for (int32_t k = 0; k < nt; k += per) v.push_back(make_int4(t0 + k, std::min(per, nt - k), -1, 0));
       far_items.push_back(v);
     }
   }
@@ -764,6 +772,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->p.ncellwork = e->p.nwork + 32;
     e->p.nlbuild = e->p.nwork + 40;
     e->p.gsort = e->p.nwork + 41;
+    e->p.tilesplit = e->p.nwork + 42;
     e->p.nitems = e->p.nwork + 48;
     e->p.nitems_far = e->p.nwork + 56;
     e->p.nitems_big = e->p.nwork + 60;
@@ -788,6 +797,8 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nitems, &n_lds_items, 4, hipMemcpyHostToDevice));
+  const int32_t split = e->tile_split ? 1 : 0;
+  HIPCK(hipMemcpy(e->p.tilesplit, &split, 4, hipMemcpyHostToDevice));
   e->cur_nwork = nw;
   e->cur_nsys = ns;
   e->cur_nitems = n_lds_items;

@@ -1121,7 +1121,7 @@ __device__ __forceinline__ float4 partner(const float4 *__restrict__ pin, const 
 template <int LPB, int MODE, bool LDS, bool IDX16, bool ONE>
 __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int s, int t, int kstep,
                                           const float4 *__restrict__ pin, float4 *__restrict__ pout,
-                                          const float4 *s_pos, int lds0) {
+                                          const float4 *s_pos, int lds0, int r0, int r1) {
   constexpr int GPW = 64 / LPB;
   const int lane = threadIdx.x & 63;
   const int g = lane / LPB, l = lane % LPB;
@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   constexpr int E = IDX16 ? 8 : 4;
   using chunk_t = uint4;
   float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;  // force on bead b0 + lane
-  for (int r = 0; r < LPB; ++r) {
+  for (int r = r0; r < r1; ++r) {
     const int li = __shfl(ordm, r * GPW + g);
     const int i = b0 + li;
     const int ns = __shfl(t_ns, li), nc = __shfl(t_nc, li), nr = __shfl(t_nr, li);
@@ -1277,6 +1277,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   }
   const int i = b0 + lane;
   if (i >= bend) return;
+  if (rm < r0 || rm >= r1) return;  // the other wave of a split tile computed this bead
   if (MODE == MODE_FORCE) {
     p.fout[i] = make_float4(mfx, mfy, mfz, 0.0f);
     return;
@@ -1348,6 +1349,11 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: system data in scalar registers
   int w0, w1;
   item_range<KLDS>(p, tier, w0, w1);
+  // split tiles: waves 2k and 2k + 1 share tile k of the item, rounds
+  // [0, LPB/2) and [LPB/2, LPB)
+  const bool split = __builtin_amdgcn_readfirstlane(*p.tilesplit) != 0;
+  const int tw = split ? wave >> 1 : wave;
+  const int r0 = split ? (wave & 1) * (LPB / 2) : 0, r1 = split ? r0 + LPB / 2 : LPB;
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1360,8 +1366,8 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
       }
       __syncthreads();
     }
-    if (wave >= it.y) continue;
-    const int t = it.x + wave;
+    if (tw >= it.y) continue;
+    const int t = it.x + tw;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
     const int par = S.parity + kstep;
@@ -1370,16 +1376,16 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
     if (MODE == MODE_STEP && S.step + kstep >= S.step_end) {
       // finished inside this block of steps: carry the coordinates over
       const int i = t * TILE + (tid & 63);
-      if (i < S.bead_off + S.n) pout[i] = pin[i];
+      if (r0 == 0 && i < S.bead_off + S.n) pout[i] = pin[i];
       continue;
     }
     const int lds0 = KLDS ? it.z : 0;
     if (S.type_mask != 3) {
-      if (S.idx16) wave_tile<LPB, MODE, KLDS, true, true>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
-      else wave_tile<LPB, MODE, KLDS, false, true>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
+      if (S.idx16) wave_tile<LPB, MODE, KLDS, true, true>(p, S, s, t, kstep, pin, pout, s_pos, lds0, r0, r1);
+      else wave_tile<LPB, MODE, KLDS, false, true>(p, S, s, t, kstep, pin, pout, s_pos, lds0, r0, r1);
     } else {
-      if (S.idx16) wave_tile<LPB, MODE, KLDS, true, false>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
-      else wave_tile<LPB, MODE, KLDS, false, false>(p, S, s, t, kstep, pin, pout, s_pos, lds0);
+      if (S.idx16) wave_tile<LPB, MODE, KLDS, true, false>(p, S, s, t, kstep, pin, pout, s_pos, lds0, r0, r1);
+      else wave_tile<LPB, MODE, KLDS, false, false>(p, S, s, t, kstep, pin, pout, s_pos, lds0, r0, r1);
     }
   }
 }
@@ -1593,6 +1599,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
                                     : nullptr;
   int w0, w1;
   item_range<KLDS>(p, tier, w0, w1);
+  const bool split = __builtin_amdgcn_readfirstlane(*p.tilesplit) != 0;
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1605,8 +1612,11 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
       }
       __syncthreads();
     }
-    if (wave >= it.y) continue;
-    const int t = it.x + wave;
+    // split tiles: the even wave of each pair filters the whole tile (the
+    // tile order it computes first must come from one wave)
+    const int tw = split ? wave >> 1 : wave;
+    if ((split && (wave & 1)) || tw >= it.y) continue;
+    const int t = it.x + tw;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
     if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;

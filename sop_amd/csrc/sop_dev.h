@@ -136,6 +136,7 @@ struct DevPtrs {
   int32_t *ncellwork;   // total home cells of the direct pair-list build
   int32_t *nlbuild;     // 1 if a system of the batch materializes its neighbor list at this rebuild
   int32_t *gsort;       // nonzero if a grid build of this rebuild sorts in global memory (too big for LDS)
+  int32_t *tilesplit;   // 1: two waves per tile of every item, each taking half the step's rounds
   int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
   int4 *items;          // step-kernel work: {first tile, tiles, first staged bead or -1, staged beads}
   int32_t *nitems;       // items that stage their systems in LDS: items[0 .. *nitems)

@@ -32,3 +32,20 @@ def test_tiers_are_bitwise_identical():
         fa = a.force_eval(ia[1])
         fb = b.force_eval(ib[1])
         assert np.array_equal(fa, fb)
+
+
+def test_split_and_whole_tiles_are_bitwise_identical():
+    # a small batch runs each 64-bead tile on two waves, half of the rounds
+    # each; a batch of more tiles than 16 per compute unit (4,536 here) runs
+    # one wave per tile.  Each bead is computed the same way either way.
+    s = S.make_chain_system(41, 4000)
+    with S.Engine(0) as a, S.Engine(0) as b:
+        ia = a.add(s)
+        ib = [b.add(s) for _ in range(72)]
+        print("split vs whole tiles: 60 steps", flush=True)
+        a.run([ia], 60)
+        b.run(ib, 60)
+        xa = a.coords(ia)
+        for k in (0, 35, 71):
+            assert np.array_equal(xa, b.coords(ib[k]))
+        assert np.array_equal(a.force_eval(ia), b.force_eval(ib[5]))
