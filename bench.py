@@ -60,7 +60,7 @@ def algorithmic_bytes(eng, ids, systems, n_pl, n_nl):
     return step, full / beads
 
 
-def cpu_baseline(systems, steps=1000, warm=20, max_threads=16, every=8):
+def cpu_baseline(systems, steps=3000, warm=20, max_threads=16, every=8):
     from concurrent.futures import ThreadPoolExecutor
 
     import sop_amd as S
@@ -95,7 +95,7 @@ def cpu_baseline(systems, steps=1000, warm=20, max_threads=16, every=8):
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Keyed by (kernel, replicas, beads);
 # any other run reports null.
 TRAFFIC = {
-    ("k_step<4,0>", 512, 737499): 396.44e6 + 20.19e6,
+    ("k_step<4,0>", 512, 737499): 328.69e6 + 11.80e6,
 }
 
 
