@@ -550,13 +550,11 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     const float r0 = d->nat_r0[k];
     skin = std::min(skin, cutoff_host(P.nl_factor, P.nl_abs, r0) - cutoff_host(P.pl_factor, P.pl_abs, r0));
   }
-  S.cell_min_edge_pl = std::max(1.0e-3f, max_rcp * 1.001f);
   S.skin = skin > 0.0f ? skin * (1.0f - 1.0e-5f) : 0.0f;
   // pair-list shell: rows within rc_pl + shell of pos_shell, valid while
   // 2 max|x - pos_shell| < shell_ok (the shell less a 1e-4 margin, so the
   // rounding of the shell cutoffs cannot matter)
   const float shell = e->opts.pl_shell;
-  S.shell = shell;
   S.shell_ok = shell * (1.0f - 1.0e-4f);
   for (int q = 0; q < 4; ++q) {
     const int ti = q / 2 + 1, tj = q % 2 + 1;
@@ -670,10 +668,20 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   // when twice the tiles still fit the wave slots of the LDS launch (two
   // blocks of STEP_WAVES per compute unit); the step then takes half the
   // latency of one tile instead of leaving waves idle
-  int64_t total_tiles = 0;
-  for (int32_t q = 0; q < nslots; ++q) total_tiles += e->sys[slots[q]].bead_len / TILE;
-  e->tile_split = total_tiles <= (int64_t)e->num_cu * STEP_WAVES;
-  const int32_t per = e->tile_split ? STEP_WAVES / 2 : STEP_WAVES;  // tiles per item
+  int64_t total_tiles = 0, far_tiles = 0;
+  for (int32_t q = 0; q < nslots; ++q) {
+    const SysHost &h = e->sys[slots[q]];
+    total_tiles += h.bead_len / TILE;
+    if (h.bead_len > lds_cap && h.bead_len > (e->fixed_grids ? 0 : e->lds_big_cap)) far_tiles += h.bead_len / TILE;
+  }
+  const int64_t slots_w = (int64_t)e->num_cu * STEP_WAVES;  // tiles of one wave each per 16 waves per unit
+  const int32_t split = total_tiles <= slots_w ? 2 : 1;
+  // the global-gather tier (slower tiles, beside the LDS launch) splits four
+  // ways when that still fits the wave slots of both launches
+  const int32_t split_far = (split == 2 && (total_tiles - far_tiles) * 2 + far_tiles * 4 <= 2 * slots_w) ? 4 : split;
+  e->tile_split[0] = e->tile_split[1] = split;
+  e->tile_split[2] = split_far;
+  const int32_t per = STEP_WAVES / split, per_far = STEP_WAVES / split_far;  // tiles per item
   std::vector<int4> lds_items, big_items;
   std::vector<std::vector<int4>> far_items;  // per non-LDS system
   size_t lds_big = 0;
@@ -711,7 +719,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     } else {
       flush();
       std::vector<int4> v;
-      for (int32_t k = 0; k < nt; k += per) v.push_back(make_int4(t0 + k, std::min(per, nt - k), -1, 0));
+      for (int32_t k = 0; k < nt; k += per_far) v.push_back(make_int4(t0 + k, std::min(per_far, nt - k), -1, 0));
       far_items.push_back(v);
     }
   }
@@ -772,7 +780,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->p.ncellwork = e->p.nwork + 32;
     e->p.nlbuild = e->p.nwork + 40;
     e->p.gsort = e->p.nwork + 41;
-    e->p.tilesplit = e->p.nwork + 42;
+    e->p.tilesplit = e->p.nwork + 44;  // 3 entries
     e->p.nitems = e->p.nwork + 48;
     e->p.nitems_far = e->p.nwork + 56;
     e->p.nitems_big = e->p.nwork + 60;
@@ -797,8 +805,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(e->p.nitems, &n_lds_items, 4, hipMemcpyHostToDevice));
-  const int32_t split = e->tile_split ? 1 : 0;
-  HIPCK(hipMemcpy(e->p.tilesplit, &split, 4, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(e->p.tilesplit, e->tile_split, sizeof(e->tile_split), hipMemcpyHostToDevice));
   e->cur_nwork = nw;
   e->cur_nsys = ns;
   e->cur_nitems = n_lds_items;

@@ -93,7 +93,7 @@ struct sop_engine {
   size_t lds_limit = 0;   // largest dynamic LDS the step kernel may use
   int64_t lds_big_cap = 0; // largest system (beads) of the whole-CU LDS tier; 0: no such tier
   int32_t step_blocks = 0; // step-kernel grid (1024-thread blocks); 0 = 2 per compute unit
-  bool tile_split = false;  // current batch: two waves per tile (half the rounds each)
+  int32_t tile_split[3] = {1, 1, 1};  // current batch: waves per tile of the LDS, whole-CU and far items
   bool fixed_grids = false; // serving: launch shapes independent of the batch, so
                             // one captured graph serves every batch composition
 

@@ -1349,11 +1349,11 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: system data in scalar registers
   int w0, w1;
   item_range<KLDS>(p, tier, w0, w1);
-  // split tiles: waves 2k and 2k + 1 share tile k of the item, rounds
-  // [0, LPB/2) and [LPB/2, LPB)
-  const bool split = __builtin_amdgcn_readfirstlane(*p.tilesplit) != 0;
-  const int tw = split ? wave >> 1 : wave;
-  const int r0 = split ? (wave & 1) * (LPB / 2) : 0, r1 = split ? r0 + LPB / 2 : LPB;
+  // split tiles: waves ns*k .. ns*k + ns - 1 share tile k of the item, each
+  // taking LPB / ns consecutive rounds
+  const int ns = __builtin_amdgcn_readfirstlane(p.tilesplit[KLDS ? tier : 2]);
+  const int tw = wave / ns;
+  const int r0 = (wave % ns) * (LPB / ns), r1 = r0 + LPB / ns;
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1599,7 +1599,6 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
                                     : nullptr;
   int w0, w1;
   item_range<KLDS>(p, tier, w0, w1);
-  const bool split = __builtin_amdgcn_readfirstlane(*p.tilesplit) != 0;
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1612,11 +1611,10 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
       }
       __syncthreads();
     }
-    // split tiles: the even wave of each pair filters the whole tile (the
-    // tile order it computes first must come from one wave)
-    const int tw = split ? wave >> 1 : wave;
-    if ((split && (wave & 1)) || tw >= it.y) continue;
-    const int t = it.x + tw;
+    // one wave per tile, split or not (the tile order it computes first must
+    // come from one wave)
+    if (wave >= it.y) continue;
+    const int t = it.x + wave;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
     if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;

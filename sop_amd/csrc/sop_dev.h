@@ -60,12 +60,12 @@ struct SysDev {
   double nl_factor, pl_factor;
   float nl_abs, pl_abs;
   float cell_min_edge;    // >= largest neighbor-list cutoff, with a 0.1% margin
-  float cell_min_edge_pl; // >= largest non-native pair-list cutoff, with a 0.1% margin
   float skin;             // smallest (rc_nl - rc_pl) over pair kinds, less a 1e-5 margin
   // pair-list shell: shell rows hold the master pairs within rc_pl + shell of
   // pos_shell; while every bead has moved less than shell_ok / 2 since then,
-  // the pair list is the shell rows filtered at x (BUILD_SHELL)
-  float shell, shell_ok;
+  // the pair list is the shell rows filtered at x (BUILD_SHELL); shell_ok is
+  // the shell width less a 1e-4 margin
+  float shell_ok;
   float rc2_sl_rep[4];    // shell-row cutoffs squared, (rc_pl + shell)^2
   float cell_min_edge_sl; // >= largest shell-row cutoff, with a 0.1% margin
   int32_t sl_valid;       // 1: the shell rows match pos_shell
@@ -136,7 +136,7 @@ struct DevPtrs {
   int32_t *ncellwork;   // total home cells of the direct pair-list build
   int32_t *nlbuild;     // 1 if a system of the batch materializes its neighbor list at this rebuild
   int32_t *gsort;       // nonzero if a grid build of this rebuild sorts in global memory (too big for LDS)
-  int32_t *tilesplit;   // 1: two waves per tile of every item, each taking half the step's rounds
+  int32_t *tilesplit;   // waves per tile (1, 2 or 4; each takes that share of the rounds) of LDS, whole-CU, far items
   int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
   int4 *items;          // step-kernel work: {first tile, tiles, first staged bead or -1, staged beads}
   int32_t *nitems;       // items that stage their systems in LDS: items[0 .. *nitems)
