@@ -1318,6 +1318,37 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&p.sys[s].err, ERR_NAN);
 }
 
+// Stage an item's arena beads [z, z + w) (whole tiles) into LDS.  Wave v
+// copies tiles v, v + STEP_WAVES, ...: lane j first looks up the coordinate
+// buffer of tile v + STEP_WAVES * j (one dependent chain per wave instead of
+// one per bead), then each batch of copies issues all its loads before its
+// LDS writes.
+__device__ __forceinline__ void stage_item(const DevPtrs &p, int z, int w, int kstep, float4 *s_pos, int wave,
+                                           int lane) {
+  const int t0 = z / TILE, nt = w / TILE;
+  const int mt = wave + STEP_WAVES * lane;
+  int par = 0;
+  if (mt < nt) par = p.sys[p.tile_sys[t0 + mt]].parity + kstep;
+  constexpr int JB = 5;
+  if (wave >= nt) return;
+  const int jl = (nt - 1 - wave) / STEP_WAVES;  // this wave's last tile is wave + STEP_WAVES * jl
+  for (int j0 = 0; j0 <= jl; j0 += JB) {
+    // no conditions, so every load issues before the first write: past its
+    // last tile the wave copies that tile again (same data, same buffer)
+    float4 v[JB];
+    int k[JB];
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      const int jj = min(j0 + j, jl);
+      k[j] = wave + STEP_WAVES * jj;
+      const int pj = __shfl(par, jj);
+      v[j] = buf_in(p, pj)[z + k[j] * TILE + lane];
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) s_pos[k[j] * TILE + lane] = v[j];
+  }
+}
+
 // Work items of one launch: LDS items (tier 0), whole-CU LDS items (tier 1),
 // or the items that gather from global memory (KLDS = false).
 template <bool KLDS>
@@ -1359,11 +1390,7 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
     if (it.x < 0) continue;
     if (KLDS) {
       __syncthreads();  // the previous item's reads are done
-      for (int b = tid; b < it.w; b += STEP_THREADS) {
-        const int a = it.z + b;  // 64 consecutive beads per wave: one tile, one system
-        const SysDev &Sb = p.sys[p.tile_sys[a / TILE]];
-        s_pos[b] = buf_in(p, Sb.parity + kstep)[a];
-      }
+      stage_item(p, it.z, it.w, kstep, s_pos, wave, tid & 63);
       __syncthreads();
     }
     if (tw >= it.y) continue;
@@ -1604,11 +1631,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
     if (it.x < 0) continue;
     if (KLDS) {
       __syncthreads();
-      for (int b = tid; b < it.w; b += STEP_THREADS) {
-        const int a = it.z + b;
-        const SysDev &Sb = p.sys[p.tile_sys[a / TILE]];
-        s_pos[b] = buf_in(p, Sb.parity + kstep)[a];
-      }
+      stage_item(p, it.z, it.w, kstep, s_pos, wave, tid & 63);
       __syncthreads();
     }
     // one wave per tile, split or not (the tile order it computes first must
