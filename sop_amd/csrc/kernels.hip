@@ -226,20 +226,26 @@ __global__ __launch_bounds__(SKIN_THREADS) void k_skin(DevPtrs p, int kstep, int
     const bool shell_chk = pl && S.sl_valid;
     float d2max = 0.0f, d2sh = 0.0f;
     if (nl || pl) {
-      for (int b = tid; b < n; b += SKIN_THREADS) {
-        const int i = bo + b;
-        const float4 xi = x[i];
-        if (nl) {
-          p.pos_ref[i] = xi;
-        } else {
-          const float4 r = p.pos_ref[i];
-          const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
-          d2max = fmaxf(d2max, dx * dx + dy * dy + dz * dz);
-        }
-        if (shell_chk) {
-          const float4 r = p.pos_shell[i];
-          const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
-          d2sh = fmaxf(d2sh, dx * dx + dy * dy + dz * dz);
+      // four beads per pass with no branch on the bead count, so the loads of
+      // a pass issue together (a bead past the end repeats the last one:
+      // the same snapshot store, the same displacement)
+      for (int b0 = tid; b0 < n; b0 += 4 * SKIN_THREADS) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = bo + min(b0 + u * SKIN_THREADS, n - 1);
+          const float4 xi = x[i];
+          if (nl) {
+            p.pos_ref[i] = xi;
+          } else {
+            const float4 r = p.pos_ref[i];
+            const float dx = xi.x - r.x, dy = xi.y - r.y, dz = xi.z - r.z;
+            d2max = fmaxf(d2max, dx * dx + dy * dy + dz * dz);
+          }
+          if (shell_chk) {
+            const float4 q = p.pos_shell[i];
+            const float ex = xi.x - q.x, ey = xi.y - q.y, ez = xi.z - q.z;
+            d2sh = fmaxf(d2sh, ex * ex + ey * ey + ez * ez);
+          }
         }
       }
     }
