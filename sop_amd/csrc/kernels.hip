@@ -586,12 +586,25 @@ __global__ __launch_bounds__(SORT_THREADS) void k_cell_sort_lds(DevPtrs p, int k
     const int n = S.n, nc = S.ncells, bo = S.bead_off, cb = S.cell_base;
     const float4 *src = build_src(p, S, kstep);
     for (int c = tid; c < nc; c += SORT_THREADS) s_cnt[c] = 0;
+    // this thread's beads (at most SB): every load and cell index before the
+    // first store (a load issued after a store would wait for it)
+    constexpr int SB = SORT_N_MAX / SORT_THREADS;
+    float4 xs[SB];
+    int cs[SB];
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      xs[u] = src[bo + min(tid + u * SORT_THREADS, n - 1)];
+      cs[u] = cell_index(S, xs[u]);
+    }
     __syncthreads();
-    for (int b = tid; b < n; b += SORT_THREADS) {
-      const int c = cell_index(S, src[bo + b]);
-      s_cell[b] = c;
-      p.cell_of[bo + b] = c;
-      atomicAdd(&s_cnt[c], 1);
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int b = tid + u * SORT_THREADS;
+      if (b < n) {
+        s_cell[b] = cs[u];
+        p.cell_of[bo + b] = cs[u];
+        atomicAdd(&s_cnt[cs[u]], 1);
+      }
     }
     __syncthreads();
     // exclusive scan of the counts: 4 consecutive cells per thread
@@ -631,17 +644,28 @@ __global__ __launch_bounds__(SORT_THREADS) void k_cell_sort_lds(DevPtrs p, int k
     __syncthreads();
     for (int b = tid; b < n; b += SORT_THREADS) s_mem[atomicAdd(&s_cnt[s_cell[b]], 1)] = b;
     __syncthreads();
-    for (int b = tid; b < n; b += SORT_THREADS) {
-      const int c = s_cell[b];
-      const int st = c > 0 ? s_cnt[c - 1] : 0, en = s_cnt[c];
-      int r = 0;
-      for (int m = st; m < en; ++m) r += s_mem[m] < b;
-      const int pos = st + r;
-      const int i = bo + b;
-      p.cell_pos[i] = pos;
-      p.cell_sorted[bo + pos] = i;
-      p.pos_sorted[bo + pos] = src[i];
-      p.nat_sorted[bo + pos] = make_int2(p.nat_start[i], p.nat_cnt[i]);
+    int ns[SB], nn[SB];
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int i = bo + min(tid + u * SORT_THREADS, n - 1);
+      ns[u] = p.nat_start[i];
+      nn[u] = p.nat_cnt[i];
+    }
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int b = tid + u * SORT_THREADS;
+      if (b < n) {
+        const int c = s_cell[b];
+        const int st = c > 0 ? s_cnt[c - 1] : 0, en = s_cnt[c];
+        int r = 0;
+        for (int m = st; m < en; ++m) r += s_mem[m] < b;
+        const int pos = st + r;
+        const int i = bo + b;
+        p.cell_pos[i] = pos;
+        p.cell_sorted[bo + pos] = i;
+        p.pos_sorted[bo + pos] = xs[u];
+        p.nat_sorted[bo + pos] = make_int2(ns[u], nn[u]);
+      }
     }
     __syncthreads();
   }
