@@ -1170,11 +1170,14 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   // per-bead list heads of the whole tile in one coalesced load each; a
   // round picks its beads' values up by shuffles
   int t_ns = 0, t_nc = 0, t_nr = 0;
+  float t_br = 0.0f;  // r0 of bond (b0 + lane, b0 + lane + 1)
   if (b0 + lane < bend) {
     t_ns = p.nat_start[b0 + lane];
     t_nc = p.platt_cnt[b0 + lane];
     t_nr = p.plrep_cnt[b0 + lane];
+    t_br = p.bond_r0[b0 + lane];
   }
+  const float br_before = b0 > bead_off ? p.bond_r0[b0 - 1] : 0.0f;  // bond (b0 - 1, b0)
   // the non-native rows in 16-byte chunks: E = 8 16-bit or 4 32-bit entries
   constexpr int E = IDX16 ? 8 : 4;
   using chunk_t = uint4;
@@ -1183,6 +1186,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     const int li = __shfl(ordm, r * GPW + g);
     const int i = b0 + li;
     const int ns = __shfl(t_ns, li), nc = __shfl(t_nc, li), nr = __shfl(t_nr, li);
+    // bond r0 of (i, i + 1) and (i - 1, i), from the tile's values
+    const float br_up = __shfl(t_br, li), br_low_t = __shfl(t_br, (li + 63) & 63);
+    const float br_low = li > 0 ? br_low_t : br_before;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (i < bend) {
       // first loads of the native and non-native loops, issued before any gather
@@ -1213,7 +1219,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float c;
             if (off == 1 || off == -1) {
-              const float r0 = p.bond_r0[off < 0 ? j : i];
+              const float r0 = off < 0 ? br_low : br_up;
               const float d = sqrtf(d2);
               const float dev = d - r0;
               const float gg = 1.0f - dev * dev * S.inv_R0sq;
