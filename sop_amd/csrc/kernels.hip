@@ -868,17 +868,29 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
     // stage the candidates [b0, b0 + nc), padded to a multiple of 128 with
     // coordinates no pair can reach
     auto stage = [&](int b0, int nc) {
-      const int ncp = (nc + 127) & ~127;
-      for (int ci = tid; ci < ncp; ci += 256) {
-        const int idx = b0 + ci;
-        if (ci < nc) {
-          int r = 0;
+      // CAND_MAX / 256 slots per thread, all loads before the LDS writes
+      // (slots past nc load the last candidate and store the padding)
+      constexpr int CS = CAND_MAX / 256;
+      float4 xc[CS];
+      int jc[CS];
 #pragma unroll
-          for (int q = 1; q < 9; ++q) r += s_roff[q] <= idx;
-          const int m = s_rst[r] + (idx - s_roff[r]);
-          s_cand[ci] = p.pos_sorted[bo + m];
-          s_cj[ci] = p.cell_sorted[bo + m];
-        } else {
+      for (int u = 0; u < CS; ++u) {
+        const int idx = b0 + min(tid + 256 * u, nc - 1);
+        int r = 0;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) r += s_roff[q] <= idx;
+        const int m = s_rst[r] + (idx - s_roff[r]);
+        xc[u] = p.pos_sorted[bo + m];
+        jc[u] = p.cell_sorted[bo + m];
+      }
+      const int ncp = (nc + 127) & ~127;
+#pragma unroll
+      for (int u = 0; u < CS; ++u) {
+        const int ci = tid + 256 * u;
+        if (ci < nc) {
+          s_cand[ci] = xc[u];
+          s_cj[ci] = jc[u];
+        } else if (ci < ncp) {
           s_cand[ci] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
           s_cj[ci] = -1;
         }
