@@ -921,10 +921,14 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         // partner and of the bead itself and its chain neighbours i +- 1, 2,
         // from their cells in this rebuild's grid (the row) and sorted
         // positions (the offset)
-        for (int h = wave; h < nh; h += 4) {
+        // four home beads per pass of a wave, 16 lanes each, so the dependent
+        // loads of several beads are in flight together
+        for (int t0 = 0; wave + 4 * t0 < nh; t0 += 4) {
+          const int h = wave + 4 * (t0 + (lane >> 4));
+          if (h >= nh) continue;
           const int ji = s_hj[h], ns = s_hns[h], nn = s_hnn[h];
           const int chi = __float_as_int(s_home[h].w) >> 2;
-          for (int k = lane; k < nn + 5; k += 64) {
+          for (int k = lane & 15; k < nn + 5; k += 16) {
             int cn, pos, jn;
             if (k < nn) {
               const int jp = p.nat_j[ns + k];  // native partner: its cell and place in the cell order
