@@ -305,11 +305,17 @@ __global__ __launch_bounds__(256) void k_bbox(DevPtrs p, int kstep) {
     if (!grid_build(S)) continue;
     const float4 *src = build_src(p, S, kstep);
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = tid; b < S.n; b += blockDim.x) {
-      const float4 x = src[S.bead_off + b];
-      mn[0] = fminf(mn[0], x.x); mx[0] = fmaxf(mx[0], x.x);
-      mn[1] = fminf(mn[1], x.y); mx[1] = fmaxf(mx[1], x.y);
-      mn[2] = fminf(mn[2], x.z); mx[2] = fmaxf(mx[2], x.z);
+    // four beads per pass with no branch on the count, so their loads overlap
+    // (a bead past the end repeats the last one; min and max are unchanged)
+    const int nb = S.n, bo = S.bead_off;
+    for (int b0 = tid; b0 < nb; b0 += 4 * (int)blockDim.x) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 x = src[bo + min(b0 + u * (int)blockDim.x, nb - 1)];
+        mn[0] = fminf(mn[0], x.x); mx[0] = fmaxf(mx[0], x.x);
+        mn[1] = fminf(mn[1], x.y); mx[1] = fmaxf(mx[1], x.y);
+        mn[2] = fminf(mn[2], x.z); mx[2] = fmaxf(mx[2], x.z);
+      }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -896,16 +902,29 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
         }
       }
     };
+    // the first home pass's beads load beside the candidate staging
+    int2 ni0 = make_int2(0, 0);
+    float4 xh0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int jh0 = 0;
+    if (tid < min(HOME_MAX, he - hs)) {
+      ni0 = p.nat_sorted[bo + hs + tid];
+      xh0 = p.pos_sorted[bo + hs + tid];
+      jh0 = p.cell_sorted[bo + hs + tid];
+    }
     if (!exact) stage(0, T);
     for (int h0 = hs; h0 < he; h0 += HOME_MAX) {
       const int nh = min(HOME_MAX, he - h0);
       if (tid < nh) {
-        const int2 ni = p.nat_sorted[bo + h0 + tid];
-        s_home[tid] = p.pos_sorted[bo + h0 + tid];
-        s_hj[tid] = p.cell_sorted[bo + h0 + tid];
+        if (h0 != hs) {
+          ni0 = p.nat_sorted[bo + h0 + tid];
+          xh0 = p.pos_sorted[bo + h0 + tid];
+          jh0 = p.cell_sorted[bo + h0 + tid];
+        }
+        s_home[tid] = xh0;
+        s_hj[tid] = jh0;
         s_hcnt[tid] = 0;
-        s_hns[tid] = ni.x;
-        s_hnn[tid] = ni.y;
+        s_hns[tid] = ni0.x;
+        s_hnn[tid] = ni0.y;
       }
       // candidate passes of CAND_MAX staged slots (one pass when they all fit,
       // staged above for every home pass)
