@@ -96,6 +96,25 @@ def test_admission_beside_a_block_and_after_pool_growth():
         assert np.array_equal(r["energies"], er[:6])
 
 
+def test_large_served_jobs_equal_alone():
+    # jobs above the LDS size (global gathers on the side stream, four waves
+    # per tile in a small batch) served beside small ones
+    systems = [(S.make_chain_system(620 + k, n), st) for k, (n, st) in
+               enumerate([(7000, 150), (300, 400), (12000, 100), (1500, 250)])]
+    for k, (s, _) in enumerate(systems):
+        s.params.seed = W.philox_key(11, k)
+    with S.Engine(0) as eng:
+        srv = serving.Server(eng)
+        ids = [srv.submit(s, st) for s, st in systems]
+        res = [srv.wait(j) for j in ids]
+        srv.stop()
+    for (s, st), (r, x) in zip(systems, res):
+        assert r["status"] == 0
+        xr, er = alone(s, st)
+        assert np.array_equal(x, xr), s.n
+        assert np.array_equal(r["energies"], er[:6])
+
+
 def test_admission_respects_hbm_budget():
     jobs = job_systems()
     with S.Engine(0) as eng:
