@@ -144,7 +144,7 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
       {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},    {(void **)&p.nat_r0, 4},  {(void **)&p.nlatt_j, 4},
-                   {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt_j, 4}, {(void **)&p.platt_r0, 4}};
+                   {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt, 8}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
                     {(void **)&p.cellitems, 4}};
 }
@@ -1126,8 +1126,13 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
     cnt_d = which == SOP_LIST_NL_NATIVE ? e->p.nlatt_cnt : e->p.platt_cnt;
     HIPCK(hipMemcpy(start.data(), e->p.nat_start + h.bead_off, 4 * (size_t)nb, hipMemcpyDeviceToHost));
     ent.resize((size_t)h.nat_len);
-    HIPCK(hipMemcpy(ent.data(), (which == SOP_LIST_NL_NATIVE ? e->p.nlatt_j : e->p.platt_j) + h.nat_off,
-                    4 * (size_t)h.nat_len, hipMemcpyDeviceToHost));
+    if (which == SOP_LIST_NL_NATIVE) {
+      HIPCK(hipMemcpy(ent.data(), e->p.nlatt_j + h.nat_off, 4 * (size_t)h.nat_len, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<int2> pe((size_t)h.nat_len);
+      HIPCK(hipMemcpy(pe.data(), e->p.platt + h.nat_off, 8 * (size_t)h.nat_len, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < pe.size(); ++k) ent[k] = pe[k].x;
+    }
     for (int i = 0; i < nb; ++i) start[i] -= (int32_t)h.nat_off;
   } else if (which == SOP_LIST_NL_REP || which == SOP_LIST_PL_REP) {
     const bool nl = which == SOP_LIST_NL_REP;

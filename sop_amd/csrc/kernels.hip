@@ -1099,8 +1099,7 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
         const uint64_t m = __ballot(pred);
         if (pred) {
           const int o = ns + cnt + lanes_below(m);
-          p.platt_j[o] = j;
-          p.platt_r0[o] = r0;
+          p.platt[o] = make_int2(j, __float_as_int(r0));
         }
         cnt += __popcll(m);
       }
@@ -1237,8 +1236,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       int jn = 0;
       float r0n = 0.0f;
       if (l < nc) {
-        jn = p.platt_j[ns + l];
-        r0n = p.platt_r0[ns + l];
+        const int2 en = p.platt[ns + l];  // {partner, bits of r0} in one load
+        jn = en.x;
+        r0n = __int_as_float(en.y);
       }
       const float4 xi = partner<LDS>(pin, s_pos, lds0, i);
       const int wi = __float_as_int(xi.w);
@@ -1280,8 +1280,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           const int j = jn;
           const float r0 = r0n;
           if (k + LPB < nc) {
-            jn = p.platt_j[ns + k + LPB];
-            r0n = p.platt_r0[ns + k + LPB];
+            const int2 en = p.platt[ns + k + LPB];
+            jn = en.x;
+            r0n = __int_as_float(en.y);
           }
           const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
@@ -1640,8 +1641,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
           const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
           if (keep) {
             const int o = ns + ncnt + __popc(bits & below);
-            p.platt_j[o] = nj[b];
-            p.platt_r0[o] = nr0[b];
+            p.platt[o] = make_int2(nj[b], __float_as_int(nr0[b]));
           }
           ncnt += __popc(bits);
         }
@@ -1792,8 +1792,9 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
         const int ns = p.nat_start[i];
         const int nc = p.platt_cnt[i];
         for (int k = l; k < nc; k += LPB) {
-          const int j = p.platt_j[ns + k];
-          const float r0 = p.platt_r0[ns + k];
+          const int2 pe = p.platt[ns + k];
+          const int j = pe.x;
+          const float r0 = __int_as_float(pe.y);
           const float4 xj = pin[j];
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           if (!wrapf) {

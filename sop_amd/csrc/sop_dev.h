@@ -110,7 +110,7 @@ struct DevPtrs {
   int32_t *nat_j;       // partner (arena index)
   float *nat_r0;
   int32_t *nlatt_j; float *nlatt_r0; int32_t *nlatt_cnt; // native neighbor list (same slots)
-  int32_t *platt_j; float *platt_r0; int32_t *platt_cnt; // native pair list (same slots)
+  int2 *platt; int32_t *platt_cnt;  // native pair list (same slots): {partner, bits of r0}
   int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list
   int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list: row i at pl_base +
                                                          // (i - bead_off) * pl_cap, entries int32 arena
