@@ -1233,13 +1233,11 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       const int nch = (nr + E - 1) / E;
       chunk_t q = {};
       if (l < nch) q = rowc[l];
-      int jn = 0;
-      float r0n = 0.0f;
-      if (l < nc) {
-        const int2 en = p.platt[ns + l];  // {partner, bits of r0} in one load
-        jn = en.x;
-        r0n = __int_as_float(en.y);
-      }
+      // the lane's first two native entries ({partner, bits of r0}), both
+      // loads issued before any gather
+      int2 na = make_int2(0, 0), nb2 = make_int2(0, 0);
+      if (l < nc) na = p.platt[ns + l];
+      if (l + LPB < nc) nb2 = p.platt[ns + l + LPB];
       const float4 xi = partner<LDS>(pin, s_pos, lds0, i);
       const int wi = __float_as_int(xi.w);
       const int row = ONE ? q1 : ((wi & 3) - 1) * 2;
@@ -1271,19 +1269,18 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           }
         }
       }
-      // native LJ over the native pair list, the next entry's loads ahead
+      // native LJ over the native pair list (the first two entries per lane loaded above)
       {
         // c = -12 eps u q6 (q6 - 1) with u = 1/d2, q6 = (r0^2 u)^3, as
         // fma(t, q6, -t) with t = -12 eps u q6
         const float eA = -12.0f * S.eps_att[row], eB = ONE ? eA : -12.0f * S.eps_att[row + 1];
         for (int k = l; k < nc; k += LPB) {
-          const int j = jn;
-          const float r0 = r0n;
-          if (k + LPB < nc) {
-            const int2 en = p.platt[ns + k + LPB];
-            jn = en.x;
-            r0n = __int_as_float(en.y);
-          }
+          int2 en;
+          if (k == l) en = na;
+          else if (k == l + LPB) en = nb2;
+          else en = p.platt[ns + k];
+          const int j = en.x;
+          const float r0 = __int_as_float(en.y);
           const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
           float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
