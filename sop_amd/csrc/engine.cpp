@@ -814,23 +814,26 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   return SOP_OK;
 }
 
+sop_status engine_error_status(sop_engine *e, int32_t s, int32_t err) {
+  if (!err) return SOP_OK;
+  std::ostringstream os;
+  os << "system " << s << ":";
+  sop_status st = SOP_ERR_STATE;
+  if (err & ERR_NAN) { os << " non-finite coordinates;"; st = SOP_ERR_NUMERIC; }
+  if (err & ERR_BOND) { os << " FENE bond stretched to |r-r0| >= R0;"; st = SOP_ERR_NUMERIC; }
+  if (err & ERR_NL_OVF) { os << " neighbor list over capacity;"; st = SOP_ERR_CAPACITY; }
+  if (err & ERR_PL_OVF) { os << " pair list over capacity;"; st = SOP_ERR_CAPACITY; }
+  if (err & ERR_PBC) { os << " extent within 10 A of boxl/2 (open cell grid needs the minimum image to be the identity);"; st = SOP_ERR_PBC; }
+  if (err & ERR_INTERNAL) { os << " inconsistent cell histogram;"; st = SOP_ERR_STATE; }
+  return set_error(e, st, os.str());
+}
+
 sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslots) {
   std::vector<SysDev> sd((size_t)e->cap.slots);
   HIPCK(hipMemcpy(sd.data(), e->p.sys, sizeof(SysDev) * sd.size(), hipMemcpyDeviceToHost));
   for (int q = 0; q < nslots; ++q) {
-    const int s = slots[q];
-    const int err = sd[s].err;
-    if (!err) continue;
-    std::ostringstream os;
-    os << "system " << s << ":";
-    sop_status st = SOP_ERR_STATE;
-    if (err & ERR_NAN) { os << " non-finite coordinates;"; st = SOP_ERR_NUMERIC; }
-    if (err & ERR_BOND) { os << " FENE bond stretched to |r-r0| >= R0;"; st = SOP_ERR_NUMERIC; }
-    if (err & ERR_NL_OVF) { os << " neighbor list over capacity;"; st = SOP_ERR_CAPACITY; }
-    if (err & ERR_PL_OVF) { os << " pair list over capacity;"; st = SOP_ERR_CAPACITY; }
-    if (err & ERR_PBC) { os << " extent within 10 A of boxl/2 (open cell grid needs the minimum image to be the identity);"; st = SOP_ERR_PBC; }
-    if (err & ERR_INTERNAL) { os << " inconsistent cell histogram;"; st = SOP_ERR_STATE; }
-    return set_error(e, st, os.str());
+    const sop_status st = engine_error_status(e, slots[q], sd[slots[q]].err);
+    if (st) return st;
   }
   return SOP_OK;
 }

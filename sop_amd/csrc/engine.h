@@ -131,6 +131,8 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
 sop_status engine_run_block_graph(sop_engine *e, int32_t G);
 sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps);
 sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslots);
+// Status (and error message) of system slot s from its error bits, on the host.
+sop_status engine_error_status(sop_engine *e, int32_t s, int32_t err);
 sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out);
 int grid_tiles(const sop_engine *e);
 int grid_sys(const sop_engine *e);
@@ -138,5 +140,6 @@ int grid_items(const sop_engine *e);
 int grid_items_far(const sop_engine *e);
 int grid_items_big(const sop_engine *e);
 StepGrids step_grids(const sop_engine *e);
+ListGrids list_grids(const sop_engine *e);
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d);
 } // namespace sopk

@@ -54,31 +54,39 @@ struct sop_engine::Serve {
 
 namespace {
 
-// Retire finished jobs: one energy launch for all of them, then coordinates
-// and energies back to the host, then free their device memory.
-sop_status retire(sop_engine *e, sop_engine::Serve *sv) {
+// After a block: retire the finished jobs and start the admitted ones as one
+// batch with one synchronization.  The energies of the finished jobs (the
+// admitted ones' lists are still empty: their share is cheap and unused),
+// their coordinates and energies back to the host, then the admitted jobs'
+// first neighbor and pair lists; the finished jobs' status comes from their
+// error bits before the list launch, the admitted jobs' from theirs after it.
+sop_status retire_and_start(sop_engine *e, sop_engine::Serve *sv, const std::vector<sop_engine::Job *> &added,
+                            const std::vector<int32_t> &added_slots) {
   std::vector<SysDev> sd((size_t)e->cap.slots);
   sop_status st = hip_check(e, hipMemcpy(sd.data(), e->p.sys, sizeof(SysDev) * sd.size(), hipMemcpyDeviceToHost),
                             "retire");
   if (st) return st;
   std::vector<sop_engine::Job *> keep, fin;
-  std::vector<int32_t> fin_slots;
+  std::vector<int32_t> slots;
   for (sop_engine::Job *j : sv->running) {
     const SysDev &S = sd[j->slot];
     if (S.step >= S.step_end || S.err) {
       fin.push_back(j);
-      fin_slots.push_back(j->slot);
+      slots.push_back(j->slot);
     } else {
       keep.push_back(j);
     }
   }
-  if (fin.empty()) return SOP_OK;
-  st = engine_set_batch(e, fin_slots.data(), (int32_t)fin_slots.size());
-  if (st) return st;
-  st = hip_check(e, launch_energy(e->p, grid_tiles(e), grid_sys(e), e->stream), "energy");
+  if (fin.empty() && added.empty()) return SOP_OK;
+  slots.insert(slots.end(), added_slots.begin(), added_slots.end());
+  st = engine_set_batch(e, slots.data(), (int32_t)slots.size());
   if (st) return st;
   std::vector<double> out(fin.size() * SYSOUT_N);
   std::vector<std::vector<float4>> xs(fin.size());
+  if (!fin.empty()) {
+    st = hip_check(e, launch_energy(e->p, grid_tiles(e), grid_sys(e), e->stream), "energy");
+    if (st) return st;
+  }
   for (size_t q = 0; q < fin.size(); ++q) {
     const SysDev &S = sd[fin[q]->slot];
     st = hip_check(e, hipMemcpyAsync(&out[q * SYSOUT_N], e->p.sysout + (int64_t)fin[q]->slot * SYSOUT_N,
@@ -89,12 +97,23 @@ sop_status retire(sop_engine *e, sop_engine::Serve *sv) {
                                      16 * (size_t)S.n, hipMemcpyDeviceToHost, e->stream), "coords copy");
     if (st) return st;
   }
+  if (!added.empty()) {
+    st = hip_check(e, launch_lists(e->p, list_grids(e), 0, FORCE_NL | FORCE_PL, e->stream), "first lists");
+    if (st) return st;
+  }
   st = hip_check(e, hipStreamSynchronize(e->stream), "retire sync");
   if (st) return st;
+  std::vector<SysDev> sa;
+  if (!added.empty()) {
+    sa.resize((size_t)e->cap.slots);
+    st = hip_check(e, hipMemcpy(sa.data(), e->p.sys, sizeof(SysDev) * sa.size(), hipMemcpyDeviceToHost),
+                   "admitted status");
+    if (st) return st;
+  }
   const double now = sop_clock_seconds();
   for (size_t q = 0; q < fin.size(); ++q) {
     sop_engine::Job *j = fin[q];
-    const int32_t status = sd[j->slot].err ? engine_check_errors(e, &j->slot, 1) : SOP_OK;
+    const int32_t status = engine_error_status(e, j->slot, sd[j->slot].err);
     engine_remove_system(e, j->slot);
     std::lock_guard<std::mutex> lk(sv->mu);
     for (size_t i = 0; i < xs[q].size(); ++i) {
@@ -109,6 +128,22 @@ sop_status retire(sop_engine *e, sop_engine::Serve *sv) {
     j->slot = -1;
     sv->used -= j->bytes;
   }
+  for (sop_engine::Job *j : added) {
+    const sop_status status = engine_error_status(e, j->slot, sa[j->slot].err);
+    if (status != SOP_OK) engine_remove_system(e, j->slot);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    j->t_admit = now;
+    if (status != SOP_OK) {
+      j->status = status;
+      j->done = 1;
+      j->t_done = now;
+      j->slot = -1;
+      sv->used -= j->bytes;
+      continue;
+    }
+    keep.push_back(j);
+  }
+  std::lock_guard<std::mutex> lk(sv->mu);
   sv->running.swap(keep);
   return SOP_OK;
 }
@@ -219,16 +254,24 @@ void serve_loop(sop_engine *e) {
     if (st == SOP_OK) admit(e, sv, run && sv->up.stream ? &sv->up : nullptr, added, added_slots, &deferred);
     const size_t beside = run && sv->up.stream ? added.size() : 0;
     const double ta = sop_clock_seconds();
-    // 3. retire finished jobs (waits for the block)
+    // 3. retire finished jobs and start the admitted ones (waits for the block)
     const size_t before = sv->running.size();
-    if (st == SOP_OK && run) st = retire(e, sv);
-    const size_t retired = before - sv->running.size();
+    size_t retired = 0;
+    if (st == SOP_OK && run) {
+      st = retire_and_start(e, sv, added, added_slots);
+      retired = before + added.size() - sv->running.size();
+    } else if (st == SOP_OK) {
+      st = start_jobs(e, sv, added, added_slots);
+    }
     const double t4 = sop_clock_seconds();
-    // 4. jobs that needed a pool to grow, now that the engine is idle; then the
-    // first lists of every admitted job
+    // 4. jobs that needed a pool to grow, now that the engine is idle
     const size_t n_deferred = deferred;
-    if (st == SOP_OK && deferred) admit(e, sv, nullptr, added, added_slots, &deferred);
-    if (st == SOP_OK) st = start_jobs(e, sv, added, added_slots);
+    std::vector<sop_engine::Job *> later;
+    std::vector<int32_t> later_slots;
+    if (st == SOP_OK && deferred) {
+      admit(e, sv, nullptr, later, later_slots, &deferred);
+      st = start_jobs(e, sv, later, later_slots);
+    }
     const double t5 = sop_clock_seconds();
     if (st == SOP_OK) {
       float ms = 0.0f;
@@ -236,7 +279,7 @@ void serve_loop(sop_engine *e) {
       std::lock_guard<std::mutex> lk(sv->mu);
       sop_serve_stats &c = sv->stats;
       c.host_admit_s += (ta - t3) + (t5 - t4);
-      c.jobs_admitted += (int64_t)added.size();
+      c.jobs_admitted += (int64_t)(added.size() + later.size());
       c.jobs_uploaded_beside += (int64_t)beside;
       c.jobs_deferred += (int64_t)n_deferred;
       if (run) {
