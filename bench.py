@@ -147,7 +147,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(replicas would shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "(replicas and served jobs would be divided among GPUs, one engine per GPU, with no communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     import sop_amd as S
