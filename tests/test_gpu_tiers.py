@@ -34,6 +34,24 @@ def test_tiers_are_bitwise_identical():
         assert np.array_equal(fa, fb)
 
 
+def test_tier_boundaries_are_bitwise_identical():
+    # sizes on either side of each tier's limit: 5,056 beads is the largest
+    # system staged at two blocks per compute unit and 5,057 the smallest on
+    # the whole-CU tier; 10,112 is the largest whole-CU system and 10,113
+    # gathers from global memory.  All against global gathers throughout.
+    sizes = (5056, 5057, 10112, 10113)
+    systems = [S.make_chain_system(50 + k, n) for k, n in enumerate(sizes)]
+    with S.Engine(0) as a, S.Engine(0, lds_beads_max=-1) as b:
+        ia = [a.add(s) for s in systems]
+        ib = [b.add(s) for s in systems]
+        print("tier boundaries: 60 steps", flush=True)
+        a.run(ia, 60)
+        b.run(ib, 60)
+        for n, x, y in zip(sizes, ia, ib):
+            assert np.array_equal(a.coords(x), b.coords(y)), n
+            assert np.array_equal(a.get_list(x, S.LIST_PL_REP), b.get_list(y, S.LIST_PL_REP)), n
+
+
 def test_split_and_whole_tiles_are_bitwise_identical():
     # a small batch runs each 64-bead tile on two waves, half of the rounds
     # each; a batch of more tiles than 16 per compute unit (4,536 here) runs
