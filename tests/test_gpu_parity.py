@@ -298,3 +298,61 @@ def test_timing_and_autotune_entry_points():
         lanes, grid = e.autotune(ids)
         assert lanes in (4, 8, 16) and grid > 0
         e.run(ids, 20)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
+def test_tiny_chains_match_oracle(eng, n):
+    # one bead (no terms at all), one bond, one i,i+2 term, one possible native:
+    # tiles padded far past the chain, rows of padding only
+    s = S.make_chain_system(70 + n, n)
+    sid = eng.add(s)
+    try:
+        o = Oracle(s)
+        _, lists = oracle_sets(o, s, s.xyz)
+        fr, _, st = o.forces(s.xyz, lists)
+        assert st == 0
+        f = eng.force_eval(sid)
+        assert np.abs(f - fr).max() <= 1e-5 * max(1.0, np.abs(fr).max())
+        eng.run([sid], 50)
+        a5, var = S.params_derived(s.params)
+        xr = o.run(s.xyz, 0, 50, s.params.seed, a5, var, s.params.n_nl, s.params.n_pl)
+        x = eng.coords(sid)
+        assert np.sqrt(((x - xr) ** 2).sum(1).mean()) <= 1e-4
+    finally:
+        eng.remove(sid)
+
+
+def test_malformed_systems_are_rejected(eng):
+    def chain10(**kw):
+        s = S.make_chain_system(80, 10)
+        for k, v in kw.items():
+            setattr(s, k, v)
+        return s
+
+    bad = {
+        "no beads": S.System(xyz=np.zeros((0, 3), np.float32), params=S.params_config()),
+        "native inside |i-j| <= 2": chain10(nat_i=np.array([0], np.int32), nat_j=np.array([2], np.int32),
+                                            nat_r0=np.array([5.0], np.float32)),
+        "duplicate native": chain10(nat_i=np.array([0, 0], np.int32), nat_j=np.array([5, 5], np.int32),
+                                    nat_r0=np.array([5.0, 5.0], np.float32)),
+        "native j out of range": chain10(nat_i=np.array([0], np.int32), nat_j=np.array([10], np.int32),
+                                         nat_r0=np.array([5.0], np.float32)),
+        "bead type 3": chain10(type=np.full(10, 3, np.int32)),
+        "decreasing chain ids": chain10(chain=np.array([1] * 5 + [0] * 5, np.int32)),
+    }
+    for what, s in bad.items():
+        with pytest.raises(S.SopError) as ei:
+            eng.add(s)
+        assert ei.value.status == S._abi.SOP_ERR_INVALID, what
+    s = S.make_chain_system(80, 10)
+    s.params.n_nl = 0
+    with pytest.raises(S.SopError) as ei:
+        eng.add(s)
+    assert ei.value.status == S._abi.SOP_ERR_INVALID
+    # the engine is still usable
+    sid = eng.add(S.make_chain_system(81, 10))
+    try:
+        eng.run([sid], 5)
+        assert eng.step_of(sid) == 5
+    finally:
+        eng.remove(sid)
