@@ -69,10 +69,10 @@ typedef struct sop_engine sop_engine;
 /* Force field and run parameters of one system.  Bead types are 1 or 2 as in
  * the reference's 3x3 tables (global.cpp:51-92); row/column 0 is unused. */
 typedef struct sop_params {
-  float k_bnd;    /* FENE spring constant, kcal/mol/A^2 (param.cpp:203) */
-  float R0;       /* FENE max deviation, A (param.cpp:204) */
-  float eps_ss;   /* i,i+2 soft-sphere epsilon (param.cpp:226) */
-  float sigma_ss; /* i,i+2 soft-sphere sigma (param.cpp:224) */
+  float k_bnd;    /* FENE spring constant, kcal/mol/A^2 (param.cpp:14) */
+  float R0;       /* FENE max deviation, A (param.cpp:15) */
+  float eps_ss;   /* i,i+2 soft-sphere epsilon (param.cpp:37) */
+  float sigma_ss; /* i,i+2 soft-sphere sigma (param.cpp:35) */
   float eps_att[3][3];   /* native LJ well depth, one table for energy and force */
   float eps_rep[3][3];   /* non-native repulsion epsilon (global.cpp:57-61) */
   float sigma_rep[3][3]; /* non-native sigma (global.cpp:75-79) */
@@ -85,11 +85,11 @@ typedef struct sop_params {
   float nl_abs;
   double pl_factor;
   float pl_abs;
-  int32_t n_nl;   /* neighbor-list rebuild interval, steps (nnlup, param.cpp:292) */
+  int32_t n_nl;   /* neighbor-list rebuild interval, steps (nnlup, param.cpp:103) */
   int32_t n_pl;   /* pair-list rebuild interval, steps (reference: every step) */
-  float T;        /* temperature, kcal/mol (param.cpp:281) */
-  float zeta;     /* friction, 1/tau_L (param.cpp:289) */
-  float h;        /* time step, tau_L (param.cpp:293) */
+  float T;        /* temperature, kcal/mol (param.cpp:92) */
+  float zeta;     /* friction, 1/tau_L (param.cpp:100) */
+  float h;        /* time step, tau_L (param.cpp:104; overdamped 0.02, param.cpp:252) */
   float boxl;     /* periodic box edge, A; <= 0 disables the minimum image */
   uint64_t seed;  /* Philox4x32-10 key of this system's random force stream */
   int32_t dynamics; /* SOP_DYN_OVERDAMPED or SOP_DYN_UNDERDAMPED */
@@ -135,14 +135,14 @@ typedef struct sop_engine_opts {
 } sop_engine_opts;
 
 /* --- parameters ---------------------------------------------------------- */
-/* Reference defaults (param.cpp:203-299, global.cpp:51-92; one eps table). */
+/* Reference defaults (param.cpp:14-110,252, global.cpp:51-92; one eps table). */
 sop_status sop_params_reference(sop_params *p);
 /* Defaults of the benchmark configurations: eps_h = eps_l = 1, sigma = 3.8,
  * sigma_ss = 5.4, absolute cutoffs 32 A / 16 A every 50 / 10 steps, T = 0.6,
  * dt = 0.05, zeta = 50, no periodic box. */
 sop_status sop_params_config(sop_params *p);
 /* Derived FP32 coefficients exactly as the engine uses them:
- * a5 = h/zeta (param.cpp:299), var = sqrt(2 T zeta / h) (energy.cu:158). */
+ * a5 = h/zeta (param.cpp:110,253), var = sqrt(2 T zeta / h) (energy.cu:158). */
 sop_status sop_params_derived(const sop_params *p, float *a5, float *var);
 /* Underdamped coefficients a[0..3] = a1..a4 with the reference's float/double
  * evaluation order (param.cpp:104-108): a1 = h(1 - zeta h/2), a2 = h h/2,

@@ -18,10 +18,10 @@ extern "C" {
 sop_status sop_params_reference(sop_params *p) {
   if (!p) return SOP_ERR_INVALID;
   std::memset(p, 0, sizeof(*p));
-  p->k_bnd = 20.0f;    // param.cpp:203
-  p->R0 = 2.0f;        // param.cpp:204
-  p->eps_ss = 1.0f;    // param.cpp:226
-  p->sigma_ss = 3.5f;  // param.cpp:224
+  p->k_bnd = 20.0f;    // param.cpp:14
+  p->R0 = 2.0f;        // param.cpp:15
+  p->eps_ss = 1.0f;    // param.cpp:37
+  p->sigma_ss = 3.5f;  // param.cpp:35
   // One well-depth table for energy and force: the force table of
   // energy.cu:3-7 (the energy table global.cpp:51-55 is its transpose).
   const float ea[3][3] = {{0, 0, 0}, {0, 1.0f, 0.8f}, {0, 0.8f, 0.7f}};
@@ -38,12 +38,12 @@ sop_status sop_params_reference(sop_params *p) {
   p->nl_abs = 0.0f;
   p->pl_factor = 2.5;  // sop.cu:861,910
   p->pl_abs = 0.0f;
-  p->n_nl = 50;        // param.cpp:292
+  p->n_nl = 50;        // param.cpp:103
   p->n_pl = 1;         // sop.cu:375-387 (pair list every step)
-  p->T = 0.6f;         // param.cpp:281
-  p->zeta = 0.05f;     // param.cpp:289
-  p->h = 0.02f;        // param.cpp:441 (dynamics overdamped)
-  p->boxl = 500.0f;    // param.cpp:286
+  p->T = 0.6f;         // param.cpp:92
+  p->zeta = 0.05f;     // param.cpp:100
+  p->h = 0.02f;        // param.cpp:252 (dynamics overdamped)
+  p->boxl = 500.0f;    // param.cpp:97
   p->seed = 1234;      // sop.cu:138
   return SOP_OK;
 }
