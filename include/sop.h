@@ -277,6 +277,8 @@ typedef struct sop_serve_stats {
   double mean_running;    /* running jobs per iteration, averaged */
   int64_t jobs_uploaded_beside; /* admissions uploaded while a block ran */
   int64_t jobs_deferred;  /* admissions put off until the engine was idle (a pool had to grow) */
+  int64_t bead_steps;     /* bead-steps run: sum over blocks and running jobs of beads x steps
+                             the job advanced in the block (finished or not) */
 } sop_serve_stats;
 sop_status sop_serve_stats_get(sop_engine *e, sop_serve_stats *out);
 double sop_clock_seconds(void);

@@ -48,6 +48,8 @@ def lib():
         _lib.orc_run.argtypes = [P(OrcSys), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64, C.c_float,
                                  C.c_float, C.c_int32, C.c_int32, P(C.c_double)]
         _lib.orc_run_grid.argtypes = _lib.orc_run.argtypes
+        _lib.orc_run_sample.argtypes = [P(OrcSys), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64, C.c_float,
+                                        C.c_float, C.c_int32, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double)]
         _lib.orc_port_create.argtypes = [P(OrcSys)]
         _lib.orc_run_ud.argtypes = [P(OrcSys), P(C.c_float), P(C.c_float), C.c_int64, C.c_int64, C.c_uint64,
                                     P(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
@@ -205,6 +207,17 @@ class Oracle:
         assert st == 0, st
         return out.reshape(-1, 3)
 
+    def run_sample(self, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, nup):
+        """orc_run recording (E_total, Rg^2, Q, contacts) after every nup-th
+        step; returns (final coordinates (n,3) float64, samples (m,4))."""
+        x = np.ascontiguousarray(x0, np.float32).reshape(-1)
+        out = np.zeros(3 * self.n, np.float64)
+        m = (step0 + nsteps) // nup - step0 // nup
+        smp = np.zeros((max(1, m), 4), np.float64)
+        st = lib().orc_run_sample(C.byref(self.s), _p(x, C.c_float), step0, nsteps, seed, a5, var, n_nl, n_pl,
+                                  nup, _p(out, C.c_double), _p(smp, C.c_double))
+        assert st == 0, st
+        return out.reshape(-1, 3), smp[:m]
 
     def run_ud(self, x0, step0, nsteps, seed, a, var, vupd, n_nl, n_pl, v0=None):
         """FP64 underdamped trajectory; returns (coordinates, velocities after

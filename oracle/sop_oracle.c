@@ -563,8 +563,14 @@ static int nl_build(const orc_sys *s, const float *xf, orc_lists *L, int grid) {
   }
 }
 
+/* samples (nup > 0): after each step that completes a multiple of nup steps
+ * (step counter (step0 + it + 1) % nup == 0), 4 doubles: the FP64 total
+ * energy over the current pair list at the new coordinates (what
+ * energy_eval reports between steps, energy.cu:69-78), Rg^2, Q and the
+ * native contact count (calculate_observables, sop.cu:528-611). */
 static int run_od(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps, uint64_t seed, float a5,
-                  float var, int32_t n_nl, int32_t n_pl, double *x_out, int grid) {
+                  float var, int32_t n_nl, int32_t n_pl, double *x_out, int grid, int32_t nup,
+                  double *samples) {
   int n = s->n;
   double *x = x_out;
   float *xf = (float *)malloc(sizeof(float) * 3 * (size_t)n);
@@ -591,6 +597,19 @@ static int run_od(const orc_sys *s, const float *x0, int64_t step0, int64_t nste
       orc_normals(seed, (uint32_t)i, (uint64_t)step, xi);
       for (int d = 0; d < 3; ++d) x[3 * i + d] += A5 * (f[3 * i + d] + VAR * xi[d]);
     }
+    if (nup > 0 && samples && (step + 1) % nup == 0) {
+      for (int i = 0; i < 3 * n; ++i) xf[i] = (float)x[i];
+      st = orc_forces(s, x, L.pla_k, L.pla, L.plr_i, L.plr_j, L.plr, f, e);
+      if (st) break;
+      double q, rg2;
+      int32_t c;
+      orc_observables(s, xf, x, &q, &c, &rg2);
+      samples[0] = e[4];
+      samples[1] = rg2;
+      samples[2] = q;
+      samples[3] = (double)c;
+      samples += 4;
+    }
   }
   free(xf); free(f); lists_free(&L);
   return st;
@@ -602,7 +621,7 @@ static int run_od(const orc_sys *s, const float *x0, int64_t step0, int64_t nste
 ORC_EXPORT int orc_run(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps,
                        uint64_t seed, float a5, float var, int32_t n_nl, int32_t n_pl,
                        double *x_out) {
-  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 0);
+  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 0, 0, NULL);
 }
 
 /* orc_run with the neighbor list of orc_nl_grid (open boundaries only). */
@@ -610,7 +629,18 @@ ORC_EXPORT int orc_run_grid(const orc_sys *s, const float *x0, int64_t step0, in
                             uint64_t seed, float a5, float var, int32_t n_nl, int32_t n_pl,
                             double *x_out) {
   if (s->boxl > 0.0f) return -4;
-  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 1);
+  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 1, 0, NULL);
+}
+
+/* orc_run that also records, every nup steps, the total energy, Rg^2, Q
+ * and the native contact count (4 doubles per sample, floor(nsteps / nup)
+ * samples when step0 is a multiple of nup): the long-run observables of the
+ * reference's update.out (sop.cu:394-396). */
+ORC_EXPORT int orc_run_sample(const orc_sys *s, const float *x0, int64_t step0, int64_t nsteps,
+                              uint64_t seed, float a5, float var, int32_t n_nl, int32_t n_pl,
+                              int32_t nup, double *x_out, double *samples) {
+  if (nup < 1) return -4;
+  return run_od(s, x0, step0, nsteps, seed, a5, var, n_nl, n_pl, x_out, 0, nup, samples);
 }
 
 /* Underdamped Langevin dynamics (sop.cu:421-484, coefficients

@@ -91,7 +91,8 @@ class ServeStats(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("jobs_admitted", C.c_int64), ("jobs_retired", C.c_int64),
                 ("steps_run", C.c_int64), ("gpu_block_s", C.c_double), ("host_admit_s", C.c_double),
                 ("host_batch_s", C.c_double), ("host_block_s", C.c_double), ("host_retire_s", C.c_double),
-                ("mean_running", C.c_double), ("jobs_uploaded_beside", C.c_int64), ("jobs_deferred", C.c_int64)]
+                ("mean_running", C.c_double), ("jobs_uploaded_beside", C.c_int64), ("jobs_deferred", C.c_int64),
+                ("bead_steps", C.c_int64)]
 
 
 class GenOpts(C.Structure):

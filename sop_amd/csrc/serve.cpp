@@ -26,6 +26,7 @@ struct sop_engine::Job {
   std::vector<int32_t> chain, type, nat_i, nat_j;
   sop_system_desc desc{};
   int64_t nsteps = 0;
+  int64_t done_steps = 0;  // steps of the blocks launched so far (capped at nsteps)
   int64_t bytes = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
   int32_t slot = -1;
@@ -237,9 +238,15 @@ void serve_loop(sop_engine *e) {
     sop_status st = SOP_OK;
     const bool run = !sv->running.empty();
     double t2 = t0, t3 = t0;
+    int64_t block_bead_steps = 0;
     if (run) {
       std::vector<int32_t> slots;
-      for (sop_engine::Job *j : sv->running) slots.push_back(j->slot);
+      for (sop_engine::Job *j : sv->running) {
+        slots.push_back(j->slot);
+        const int64_t adv = std::min<int64_t>(sv->block, j->nsteps - j->done_steps);
+        j->done_steps += adv;
+        block_bead_steps += (int64_t)j->desc.n_beads * adv;
+      }
       st = engine_set_batch(e, slots.data(), (int32_t)slots.size());
       t2 = sop_clock_seconds();
       if (st == SOP_OK && sv->ev0) (void)hipEventRecord(sv->ev0, e->stream);
@@ -287,6 +294,7 @@ void serve_loop(sop_engine *e) {
         c.iterations += 1;
         c.jobs_retired += (int64_t)retired;
         c.steps_run += sv->block;
+        c.bead_steps += block_bead_steps;
         c.gpu_block_s += 1e-3 * ms;
         c.host_batch_s += t2 - t0;
         c.host_block_s += t3 - t2;

@@ -66,66 +66,115 @@ class Server:
 
 
 def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates=256, seed=4,
-                 budget_gib=32.0, drain_timeout=120.0, lanes_per_bead=4, block_steps=0):
+                 budget_gib=32.0, drain_timeout=120.0, lanes_per_bead=4, block_steps=0, window=None,
+                 resident=None, resident_steps=0):
     """Served workload: Poisson arrivals at `rate` jobs/s for `seconds`; job
     sizes log-normal (median 1500, sigma 0.9) clipped to [100, 20000]; steps
     log-uniform in [steps_lo, steps_hi]; T uniform in [0.5, 0.9]; fresh Philox
     key per job.  Chains come from `templates` pre-generated compact walks of
-    the drawn sizes (job k uses template k % templates).  Returns latency
-    percentiles and aggregate bead-steps/s of the completed jobs.  The
-    engine's step kernel uses `lanes_per_bead` lanes per bead; the server
-    runs `block_steps` steps per graph replay (0: the jobs' n_nl)."""
+    the drawn sizes (job k uses template k % templates).  The engine's step
+    kernel uses `lanes_per_bead` lanes per bead; the server runs `block_steps`
+    steps per graph replay (0: the jobs' n_nl).
+
+    window=None: wait until every job is done (or drain_timeout) and report
+    latency percentiles and the completed jobs' bead-steps/s.  window=W: the
+    measurement ends W seconds after the first arrival, whatever is still
+    queued or running (an offered load above capacity never drains): the
+    server's capacity is the bead-steps it ran in the window (the serve loop's
+    own count, finished jobs or not) / W; latencies are those of the jobs
+    completed in the window, and the rest are reported as censored.
+
+    resident: a System submitted before the stream with resident_steps steps
+    (a large assembly run alongside the served stream); its steps/s from
+    admission to completion is reported."""
     jobs = W.c4_jobs(rate=rate, duration=seconds, seed=seed, steps_lo=steps_lo, steps_hi=steps_hi)
     t0 = time.perf_counter()
     base = [W.c4_system(jobs[k], seed) for k in range(min(templates, len(jobs)))]
-    log(f"served: {len(jobs)} arrivals over {seconds}s, {len(base)} chain templates in "
-        f"{time.perf_counter() - t0:.1f}s")
+    log(f"served: {len(jobs)} arrivals over {seconds}s, steps in [{steps_lo:.0f}, {steps_hi:.0f}], "
+        f"{len(base)} chain templates in {time.perf_counter() - t0:.1f}s")
     eng = Engine(0, lanes_per_bead=lanes_per_bead)
-    srv = Server(eng, hbm_budget_bytes=int(budget_gib * (1 << 30)), block_steps=block_steps)
-    ids = []
-    start = time.perf_counter()
-    clock0 = lib().sop_clock_seconds()
-    for (t, nb, nst, T, k) in jobs:
-        now = time.perf_counter() - start
-        if t > now:
-            time.sleep(t - now)
-        s = base[k % len(base)]
-        s.params.T = T
-        s.params.seed = W.philox_key(seed, k)
-        ids.append((srv.submit(s, nst), s.n, nst))
-    log(f"served: all {len(ids)} jobs submitted at {time.perf_counter() - start:.1f}s")
-    lat, done_beadsteps, t_last = [], 0, clock0
-    pending = list(ids)
-    deadline = time.perf_counter() + drain_timeout
-    while pending and time.perf_counter() < deadline:
-        keep = []
-        for (j, n, nst) in pending:
-            done, res, _ = srv.poll(j)
-            if done:
-                if res["status"] != 0:
-                    raise SopError(res["status"], f"served job {j} failed")
-                lat.append(res["t_done"] - res["t_submit"])
-                done_beadsteps += n * nst
-                t_last = max(t_last, res["t_done"])
-            else:
-                keep.append((j, n, nst))
-        pending = keep
-        if pending:
-            time.sleep(0.01)
-    loop = srv.stats()
-    srv.stop()
-    eng.close()
-    wall = t_last - clock0
+    srv = None
+    res_out = None
+    try:
+        srv = Server(eng, hbm_budget_bytes=int(budget_gib * (1 << 30)), block_steps=block_steps)
+        res_id = srv.submit(resident, resident_steps) if resident is not None else None
+        ids = []
+        start = time.perf_counter()
+        clock0 = lib().sop_clock_seconds()
+        stop_at = start + window if window is not None else None
+        for (t, nb, nst, T, k) in jobs:
+            now = time.perf_counter() - start
+            if stop_at is not None and start + t > stop_at:
+                break
+            if t > now:
+                time.sleep(t - now)
+            s = base[k % len(base)]
+            s.params.T = T
+            s.params.seed = W.philox_key(seed, k)
+            ids.append((srv.submit(s, nst), s.n, nst))
+        log(f"served: {len(ids)} jobs submitted at {time.perf_counter() - start:.1f}s")
+        lat, done_beadsteps, t_last = [], 0, clock0
+        pending = list(ids)
+        deadline = stop_at if stop_at is not None else time.perf_counter() + drain_timeout
+        res_done = res_id is None
+        while (pending or not res_done) and time.perf_counter() < deadline:
+            keep = []
+            for (j, n, nst) in pending:
+                done, r, _ = srv.poll(j)
+                if done:
+                    if r["status"] != 0:
+                        msg = lib().sop_last_error(eng._h).decode(errors="replace")
+                        raise SopError(r["status"], f"served job {j} ({n} beads, {nst} steps) failed: {msg}")
+                    lat.append(r["t_done"] - r["t_submit"])
+                    done_beadsteps += n * nst
+                    t_last = max(t_last, r["t_done"])
+                else:
+                    keep.append((j, n, nst))
+            pending = keep
+            if not res_done:
+                done, r, _ = srv.poll(res_id)
+                if done:
+                    res_done = True
+                    if r["status"] != 0:
+                        raise SopError(r["status"], "resident job failed")
+                    dt = r["t_done"] - r["t_admit"]
+                    res_out = {"n_beads": resident.n, "steps": resident_steps, "seconds": dt,
+                               "steps_per_s": resident_steps / dt, "bead_steps_per_s": resident.n * resident_steps / dt}
+            if pending or not res_done:
+                time.sleep(0.01)
+        t_end = lib().sop_clock_seconds()
+        loop = srv.stats()
+    finally:
+        # stop the serve thread before the engine goes, whatever happened above
+        try:
+            if srv is not None:
+                srv.stop()
+        except SopError as ex:
+            log(f"served: the serve loop stopped with an error: {ex}")
+        finally:
+            eng.close()
     lat = np.array(lat)
     offered = sum(n * nst for (_, n, nst) in ids) / seconds
-    out = {"jobs": len(ids), "completed": int(len(lat)), "arrival_rate": rate, "arrival_seconds": seconds,
+    out = {"jobs": len(ids), "completed": int(len(lat)), "censored": len(ids) - int(len(lat)),
+           "arrival_rate": rate, "arrival_seconds": seconds, "window_s": window,
            "steps_range": [steps_lo, steps_hi], "templates": len(base), "block_steps": block_steps,
            "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
-           "bead_steps_per_s": done_beadsteps / wall if wall > 0 else None,
-           "offered_bead_steps_per_s": offered, "wall_s": wall,
+           "offered_bead_steps_per_s": offered,
            "loop": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in loop.items()}}
-    log(f"served: {out['completed']}/{out['jobs']} done, p50 {out['p50_latency_s']:.3f}s "
-        f"p99 {out['p99_latency_s']:.3f}s, {out['bead_steps_per_s']:.4g} bead-steps/s "
-        f"(offered {offered:.4g})")
+    if window is None:
+        wall = t_last - clock0
+        out["bead_steps_per_s"] = done_beadsteps / wall if wall > 0 else None
+        out["wall_s"] = wall
+    else:
+        out["capacity_bead_steps_per_s"] = loop["bead_steps"] / (t_end - clock0)
+        out["completed_bead_steps_per_s"] = done_beadsteps / (t_end - clock0)
+        out["wall_s"] = t_end - clock0
+    if resident is not None:
+        out["resident"] = res_out
+    p50, p99 = out["p50_latency_s"], out["p99_latency_s"]
+    rate_s = out.get("bead_steps_per_s") or out.get("capacity_bead_steps_per_s")
+    log(f"served: {out['completed']}/{out['jobs']} done ({out['censored']} censored), "
+        f"p50 {p50 if p50 is None else round(p50, 3)}s p99 {p99 if p99 is None else round(p99, 3)}s, "
+        f"{rate_s:.4g} bead-steps/s (offered {offered:.4g})" + (f", resident {res_out}" if resident is not None else ""))
     return out

@@ -21,6 +21,6 @@ int main(void) {
   printf("sop_list_stats %zu\n", sizeof(sop_list_stats));
   printf("sop_gen_opts %zu\n", sizeof(sop_gen_opts));
   printf("sop_serve_stats %zu\n", sizeof(sop_serve_stats));
-  F(sop_serve_stats, gpu_block_s); F(sop_serve_stats, mean_running);
+  F(sop_serve_stats, gpu_block_s); F(sop_serve_stats, mean_running); F(sop_serve_stats, bead_steps);
   return 0;
 }

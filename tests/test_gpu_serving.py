@@ -98,9 +98,10 @@ def test_admission_beside_a_block_and_after_pool_growth():
 
 def test_large_served_jobs_equal_alone():
     # jobs above the LDS size (global gathers on the side stream, four waves
-    # per tile in a small batch) served beside small ones
+    # per tile in a small batch), up to the served stream's 20,000-bead
+    # maximum, served beside small ones
     systems = [(S.make_chain_system(620 + k, n), st) for k, (n, st) in
-               enumerate([(7000, 150), (300, 400), (12000, 100), (1500, 250)])]
+               enumerate([(7000, 150), (300, 400), (12000, 100), (1500, 250), (20000, 60)])]
     for k, (s, _) in enumerate(systems):
         s.params.seed = W.philox_key(11, k)
     with S.Engine(0) as eng:
