@@ -132,6 +132,10 @@ typedef struct sop_engine_opts {
                              pair-list rebuilds filter those rows while every bead has
                              moved less than pl_shell / 2 (same lists, less work) */
   int64_t pool_bytes;     /* serve: HBM budget for all pools; 0 = grow on demand */
+  int32_t step_tiles;     /* 64-bead tiles per work item of the step kernel: 16, 8, 4, 2 or 1;
+                             an item of t tiles runs each tile on min(lanes_per_bead, 16 / t)
+                             waves (0 = 16 while the batch fills the device, fewer below) */
+  int32_t reserved_opts;  /* 0 */
 } sop_engine_opts;
 
 /* --- parameters ---------------------------------------------------------- */

@@ -194,7 +194,7 @@ class Engine:
     """One HIP device's engine (libsop).  Not thread-safe."""
 
     def __init__(self, device=0, lanes_per_bead=4, grid_blocks=0, use_graphs=True, nl_cap_max=0,
-                 pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None):
+                 pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None, step_tiles=0):
         self._h = None
         o = _abi.EngineOpts()
         lib().sop_engine_opts_default(C.byref(o))
@@ -212,6 +212,7 @@ class Engine:
             o.lds_beads_max = lds_beads_max
         if pl_shell is not None:
             o.pl_shell = pl_shell  # < 0: no shell (every pair-list rebuild scans the cell grid)
+        o.step_tiles = step_tiles  # tiles per step-kernel item (0: by batch size)
         h = C.c_void_p()
         _check(lib().sop_engine_create(device, C.byref(o), C.byref(h)))
         self._h = h

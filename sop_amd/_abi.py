@@ -64,6 +64,7 @@ class EngineOpts(C.Structure):
         ("nl_cap_max", C.c_int32), ("pl_cap_max", C.c_int32), ("cell_cap_max", C.c_int32),
         ("lanes_per_bead", C.c_int32), ("grid_blocks", C.c_int32), ("use_graphs", C.c_int32),
         ("lds_beads_max", C.c_int32), ("pl_shell", C.c_float), ("pool_bytes", C.c_int64),
+        ("step_tiles", C.c_int32), ("reserved_opts", C.c_int32),
     ]
 
 

@@ -87,39 +87,38 @@ int grid_sys(const sop_engine *e) {
   return std::max(1, std::min(e->cur_nsys, 2048));
 }
 
-int grid_items(const sop_engine *e) {
-  int g = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
+// Launch grid of a persistent item kernel: `g` blocks (a multiple of 8, so
+// that item k runs on XCD k mod 8), fewer when the batch has fewer items, none
+// when it has none; fixed grids (serving) always launch g.
+static int items_grid(int g, bool fixed, int count) {
   g = std::max(8, g / 8 * 8);
-  if (e->fixed_grids) return g;
-  if (e->cur_nitems == 0) return 0;  // no launch
-  const int need = (e->cur_nitems + 7) / 8 * 8;
-  return std::max(8, std::min(g, need));
+  if (fixed) return g;
+  if (count == 0) return 0;  // no launch
+  return std::max(8, std::min(g, (count + 7) / 8 * 8));
 }
 
-int grid_items_far(const sop_engine *e) {
-  int g = 2 * e->num_cu;
-  g = std::max(8, g / 8 * 8);
-  if (e->fixed_grids) return g;
-  if (e->cur_nitems_far == 0) return 0;  // no launch
-  const int need = (e->cur_nitems_far + 7) / 8 * 8;
-  return std::max(8, std::min(g, need));
+int grid_items(const sop_engine *e) {
+  return items_grid(e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu, e->fixed_grids, e->cur_nitems);
 }
+
+int grid_items_far(const sop_engine *e) { return items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nitems_far); }
 
 // Whole-CU LDS items: one block per compute unit (the block takes all of
-// the unit's LDS).
-int grid_items_big(const sop_engine *e) {
-  const int g = std::max(8, e->num_cu / 8 * 8);
-  if (e->cur_nitems_big == 0) return 0;  // no launch (always while serving)
-  const int need = (e->cur_nitems_big + 7) / 8 * 8;
-  return std::max(8, std::min(g, need));
-}
+// the unit's LDS); never while serving.
+int grid_items_big(const sop_engine *e) { return items_grid(e->num_cu, false, e->cur_nitems_big); }
 
 StepGrids step_grids(const sop_engine *e) {
   return StepGrids{grid_items(e), grid_items_big(e), grid_items_far(e), e->cur_lds, e->cur_lds_big};
 }
 
+StepGrids filter_grids(const sop_engine *e) {
+  return StepGrids{items_grid(e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu, e->fixed_grids, e->cur_nfitems),
+                   items_grid(e->num_cu, false, e->cur_nfitems_big),
+                   items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nfitems_far), e->cur_flds, e->cur_flds_big};
+}
+
 ListGrids list_grids(const sop_engine *e) {
-  return ListGrids{grid_tiles(e), grid_sys(e), step_grids(e)};
+  return ListGrids{grid_tiles(e), grid_sys(e), filter_grids(e)};
 }
 
 } // namespace sopk
@@ -647,11 +646,18 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   e->h_work.assign(mx * NX, -1);
   for (int x = 0; x < NX; ++x)
     for (size_t m = 0; m < bins[x].size(); ++m) e->h_work[m * NX + x] = bins[x][m];
-  // Step-kernel items of up to STEP_WAVES tiles (one per wave).  Systems that
-  // fit the LDS budget are staged whole: arena-adjacent small systems share an
-  // item (one staged range), a system of more than STEP_WAVES tiles gets one
-  // item per STEP_WAVES tiles, each staging the whole system.  Larger systems
-  // gather from global memory and keep all their items on one XCD.
+  // Work items of the step kernel and of the shell filter: {first tile, tiles
+  // (| waves per tile << 8 for the step kernel), first staged bead or -1,
+  // staged beads}.  Systems that fit the LDS budget are staged whole:
+  // arena-adjacent small systems share an item (one staged range), a larger
+  // system gets one item per `per` tiles, each staging the whole system.
+  // Larger systems gather from global memory and keep all their items on one
+  // XCD.  The shell filter takes one wave per tile, STEP_WAVES tiles per item.
+  // The step kernel's items hold `per` tiles and run each tile on
+  // min(lanes_per_bead, STEP_WAVES / tiles) waves (power of two), each wave
+  // taking that share of the tile's rounds: an item's duration shrinks with
+  // its tile count, so small items do not leave waves idle while the big
+  // ones finish.
   const int64_t lds_cap = e->opts.lds_beads_max > 0
                               ? std::min<int64_t>(e->opts.lds_beads_max, (int64_t)(e->lds_limit / 16))
                               : 0;
@@ -664,107 +670,127 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   std::vector<int32_t> by_arena(slots, slots + nslots);
   std::sort(by_arena.begin(), by_arena.end(),
             [&](int32_t a2, int32_t b2) { return e->sys[a2].bead_off < e->sys[b2].bead_off; });
-  // small batches: two waves per tile, each taking half of the tile's rounds,
-  // when twice the tiles still fit the wave slots of the LDS launch (two
-  // blocks of STEP_WAVES per compute unit); the step then takes half the
-  // latency of one tile instead of leaving waves idle
-  int64_t total_tiles = 0, far_tiles = 0;
-  for (int32_t q = 0; q < nslots; ++q) {
-    const SysHost &h = e->sys[slots[q]];
-    total_tiles += h.bead_len / TILE;
-    if (h.bead_len > lds_cap && h.bead_len > (e->fixed_grids ? 0 : e->lds_big_cap)) far_tiles += h.bead_len / TILE;
-  }
-  const int64_t slots_w = (int64_t)e->num_cu * STEP_WAVES;  // tiles of one wave each per 16 waves per unit
-  const int32_t split = total_tiles <= slots_w ? 2 : 1;
-  // the global-gather tier (slower tiles, beside the LDS launch) splits four
-  // ways when that still fits the wave slots of both launches
-  const int32_t split_far = (split == 2 && (total_tiles - far_tiles) * 2 + far_tiles * 4 <= 2 * slots_w) ? 4 : split;
-  e->tile_split[0] = e->tile_split[1] = split;
-  e->tile_split[2] = split_far;
-  const int32_t per = STEP_WAVES / split, per_far = STEP_WAVES / split_far;  // tiles per item
-  std::vector<int4> lds_items, big_items;
-  std::vector<std::vector<int4>> far_items;  // per non-LDS system
-  size_t lds_big = 0;
-  int4 cur = make_int4(-1, 0, -1, 0);
-  auto flush = [&]() {
-    if (cur.x >= 0) lds_items.push_back(cur);
-    cur = make_int4(-1, 0, -1, 0);
+  // tiles per step item: the option, or 16 while the batch's tiles fill the
+  // wave slots of the LDS launch (two blocks of STEP_WAVES per compute unit),
+  // 8 or 4 below that, so that a small batch still fills the device
+  int64_t total_tiles = 0;
+  for (int32_t q = 0; q < nslots; ++q) total_tiles += e->sys[slots[q]].bead_len / TILE;
+  const int64_t slots_w = (int64_t)e->num_cu * STEP_WAVES;
+  const int lpb = e->opts.lanes_per_bead;
+  int32_t per_step = e->opts.step_tiles;
+  if (per_step <= 0) per_step = total_tiles > slots_w ? STEP_WAVES : (2 * total_tiles > slots_w ? 8 : 4);
+  e->cur_step_tiles = per_step;
+  struct Plan {
+    std::vector<int4> items;
+    int32_t n_lds = 0, n_big = 0, n_far = 0;
+    size_t lds = 0, lds_big = 0;
   };
-  size_t lds = 0;
-  for (int32_t sl : by_arena) {
-    const SysHost &h = e->sys[sl];
-    const int32_t t0 = (int32_t)(h.bead_off / TILE), nt = (int32_t)(h.bead_len / TILE);
-    if (h.bead_len <= lds_cap) {
-      lds = std::max(lds, (size_t)h.bead_len * 16);
-      if (nt > per) {
+  auto plan = [&](int32_t per, bool split, Plan &out) {
+    // waves per tile of an item of nt tiles
+    auto waves = [&](int32_t nt) {
+      int32_t w = 1;
+      if (split)
+        while (w * 2 <= lpb && nt * w * 2 <= STEP_WAVES) w *= 2;
+      return w;
+    };
+    auto mk = [&](int32_t t0, int32_t nt, int32_t z, int32_t wlen) {
+      return make_int4(t0, nt | (waves(nt) << 8), z, wlen);
+    };
+    std::vector<int4> lds_items, big_items;
+    std::vector<std::vector<int4>> far_items;  // per non-LDS system
+    int4 cur = make_int4(-1, 0, -1, 0);        // .y: tiles while the item grows
+    auto flush = [&]() {
+      if (cur.x >= 0) lds_items.push_back(mk(cur.x, cur.y, cur.z, cur.w));
+      cur = make_int4(-1, 0, -1, 0);
+    };
+    for (int32_t sl : by_arena) {
+      const SysHost &h = e->sys[sl];
+      const int32_t t0 = (int32_t)(h.bead_off / TILE), nt = (int32_t)(h.bead_len / TILE);
+      if (h.bead_len <= lds_cap) {
+        out.lds = std::max(out.lds, (size_t)h.bead_len * 16);
+        if (nt > per) {
+          flush();
+          for (int32_t k = 0; k < nt; k += per)
+            lds_items.push_back(mk(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
+          continue;
+        }
+        if (cur.x >= 0 && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= per &&
+            (int64_t)cur.w + h.bead_len <= lds_cap) {
+          cur.y += nt;
+          cur.w += (int32_t)h.bead_len;
+          out.lds = std::max(out.lds, (size_t)cur.w * 16);
+        } else {
+          flush();
+          cur = make_int4(t0, nt, (int32_t)h.bead_off, (int32_t)h.bead_len);
+        }
+      } else if (h.bead_len <= big_cap) {
         flush();
+        out.lds_big = std::max(out.lds_big, (size_t)h.bead_len * 16);
         for (int32_t k = 0; k < nt; k += per)
-          lds_items.push_back(make_int4(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
-        continue;
-      }
-      if (cur.x >= 0 && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= per &&
-          (int64_t)cur.w + h.bead_len <= lds_cap) {
-        cur.y += nt;
-        cur.w += (int32_t)h.bead_len;
-        lds = std::max(lds, (size_t)cur.w * 16);
+          big_items.push_back(mk(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
       } else {
         flush();
-        cur = make_int4(t0, nt, (int32_t)h.bead_off, (int32_t)h.bead_len);
+        std::vector<int4> v;
+        for (int32_t k = 0; k < nt; k += per) v.push_back(mk(t0 + k, std::min(per, nt - k), -1, 0));
+        far_items.push_back(v);
       }
-    } else if (h.bead_len <= big_cap) {
-      flush();
-      lds_big = std::max(lds_big, (size_t)h.bead_len * 16);
-      for (int32_t k = 0; k < nt; k += per)
-        big_items.push_back(make_int4(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
-    } else {
-      flush();
-      std::vector<int4> v;
-      for (int32_t k = 0; k < nt; k += per_far) v.push_back(make_int4(t0 + k, std::min(per_far, nt - k), -1, 0));
-      far_items.push_back(v);
     }
-  }
-  flush();
-  // XCD bins by tile load (item k runs on XCD k mod 8): the LDS items, then
-  // whole non-LDS systems (largest first) so each keeps its gathers in one L2
-  std::vector<std::vector<int4>> ibins(NX), bbins(NX), fbins(NX);
-  std::vector<int64_t> iload(NX, 0), bload(NX, 0), fload(NX, 0);
-  std::stable_sort(lds_items.begin(), lds_items.end(), [](const int4 &u, const int4 &v) { return u.y > v.y; });
-  for (const int4 &it : lds_items) {
-    const int x = (int)(std::min_element(iload.begin(), iload.end()) - iload.begin());
-    ibins[x].push_back(it);
-    iload[x] += it.y;
-  }
-  std::stable_sort(big_items.begin(), big_items.end(), [](const int4 &u, const int4 &v) { return u.y > v.y; });
-  for (const int4 &it : big_items) {
-    const int x = (int)(std::min_element(bload.begin(), bload.end()) - bload.begin());
-    bbins[x].push_back(it);
-    bload[x] += it.y;
-  }
-  std::sort(far_items.begin(), far_items.end(), [](const std::vector<int4> &u, const std::vector<int4> &v) {
-    return u.size() != v.size() ? u.size() > v.size() : u[0].x < v[0].x;
-  });
-  for (const auto &v : far_items) {
-    const int x = (int)(std::min_element(fload.begin(), fload.end()) - fload.begin());
-    for (const int4 &it : v) {
-      fbins[x].push_back(it);
-      fload[x] += it.y;
+    flush();
+    // XCD bins by load (item k runs on XCD k mod 8): the LDS items, longest
+    // first (an item lasts about one tile's rounds / its waves per tile), then
+    // whole non-LDS systems (largest first) so each keeps its gathers in one L2
+    std::vector<std::vector<int4>> ibins(NX), bbins(NX), fbins(NX);
+    std::vector<double> iload(NX, 0), bload(NX, 0), fload(NX, 0);
+    auto by_dur = [&](const int4 &u, const int4 &v) {
+      const double du = 1.0 / item_split(u.y), dv = 1.0 / item_split(v.y);
+      return du != dv ? du > dv : item_tiles(u.y) > item_tiles(v.y);
+    };
+    std::stable_sort(lds_items.begin(), lds_items.end(), by_dur);
+    for (const int4 &it : lds_items) {
+      const int x = (int)(std::min_element(iload.begin(), iload.end()) - iload.begin());
+      ibins[x].push_back(it);
+      iload[x] += (double)item_tiles(it.y);
     }
-  }
-  size_t imx = 0, bmx = 0, fmx = 0;
-  for (auto &bn : ibins) imx = std::max(imx, bn.size());
-  for (auto &bn : bbins) bmx = std::max(bmx, bn.size());
-  for (auto &bn : fbins) fmx = std::max(fmx, bn.size());
-  e->h_items.assign((imx + bmx + fmx) * NX, make_int4(-1, 0, -1, 0));
-  for (int x = 0; x < NX; ++x) {
-    for (size_t m = 0; m < ibins[x].size(); ++m) e->h_items[m * NX + x] = ibins[x][m];
-    for (size_t m = 0; m < bbins[x].size(); ++m) e->h_items[(imx + m) * NX + x] = bbins[x][m];
-    for (size_t m = 0; m < fbins[x].size(); ++m) e->h_items[(imx + bmx + m) * NX + x] = fbins[x][m];
-  }
-  const int32_t n_lds_items = (int32_t)(imx * NX), n_big_items = (int32_t)(bmx * NX),
-                n_far_items = (int32_t)(fmx * NX);
-  if (e->fixed_grids && lds_cap > 0) lds = (size_t)lds_cap * 16;
-  e->cur_lds = lds;
-  e->cur_lds_big = lds_big;
+    std::stable_sort(big_items.begin(), big_items.end(), by_dur);
+    for (const int4 &it : big_items) {
+      const int x = (int)(std::min_element(bload.begin(), bload.end()) - bload.begin());
+      bbins[x].push_back(it);
+      bload[x] += (double)item_tiles(it.y);
+    }
+    std::sort(far_items.begin(), far_items.end(), [](const std::vector<int4> &u, const std::vector<int4> &v) {
+      return u.size() != v.size() ? u.size() > v.size() : u[0].x < v[0].x;
+    });
+    for (const auto &v : far_items) {
+      const int x = (int)(std::min_element(fload.begin(), fload.end()) - fload.begin());
+      for (const int4 &it : v) {
+        fbins[x].push_back(it);
+        fload[x] += (double)item_tiles(it.y);
+      }
+    }
+    size_t imx = 0, bmx = 0, fmx = 0;
+    for (auto &bn : ibins) imx = std::max(imx, bn.size());
+    for (auto &bn : bbins) bmx = std::max(bmx, bn.size());
+    for (auto &bn : fbins) fmx = std::max(fmx, bn.size());
+    out.items.assign((imx + bmx + fmx) * NX, make_int4(-1, 0, -1, 0));
+    for (int x = 0; x < NX; ++x) {
+      for (size_t m = 0; m < ibins[x].size(); ++m) out.items[m * NX + x] = ibins[x][m];
+      for (size_t m = 0; m < bbins[x].size(); ++m) out.items[(imx + m) * NX + x] = bbins[x][m];
+      for (size_t m = 0; m < fbins[x].size(); ++m) out.items[(imx + bmx + m) * NX + x] = fbins[x][m];
+    }
+    out.n_lds = (int32_t)(imx * NX);
+    out.n_big = (int32_t)(bmx * NX);
+    out.n_far = (int32_t)(fmx * NX);
+    if (e->fixed_grids && lds_cap > 0) out.lds = (size_t)lds_cap * 16;
+  };
+  Plan sp, fp;
+  plan(per_step, true, sp);
+  plan(STEP_WAVES, false, fp);
+  e->h_items.swap(sp.items);
+  e->h_fitems.swap(fp.items);
+  e->cur_lds = sp.lds;
+  e->cur_lds_big = sp.lds_big;
+  e->cur_flds = fp.lds;
+  e->cur_flds_big = fp.lds_big;
   e->h_syslist.assign(slots, slots + nslots);
   if ((int64_t)e->h_work.size() + 2 > e->work_cap) {
     HIPCK(hipStreamSynchronize(e->stream));
@@ -780,7 +806,9 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->p.ncellwork = e->p.nwork + 32;
     e->p.nlbuild = e->p.nwork + 40;
     e->p.gsort = e->p.nwork + 41;
-    e->p.tilesplit = e->p.nwork + 44;  // 3 entries
+    e->p.nfitems = e->p.nwork + 44;
+    e->p.nfitems_big = e->p.nwork + 45;
+    e->p.nfitems_far = e->p.nwork + 46;
     e->p.nitems = e->p.nwork + 48;
     e->p.nitems_far = e->p.nwork + 56;
     e->p.nitems_big = e->p.nwork + 60;
@@ -794,23 +822,42 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     e->items_cap = nc;
     invalidate_graphs(e);
   }
+  if ((int64_t)e->h_fitems.size() + 1 > e->fitems_cap) {
+    HIPCK(hipStreamSynchronize(e->stream));
+    const int64_t nc = std::max<int64_t>(1024, (int64_t)e->h_fitems.size() * 2 + 1);
+    if (e->p.fitems) HIPCK(hipFree(e->p.fitems));
+    HIPCK(hipMalloc((void **)&e->p.fitems, sizeof(int4) * (size_t)nc));
+    e->fitems_cap = nc;
+    invalidate_graphs(e);
+  }
   if (nslots > e->cap.slots) return set_error(e, SOP_ERR_INVALID, "batch larger than slot capacity");
   const int32_t nw = (int32_t)e->h_work.size(), ns = nslots;
   if (nw > 0) HIPCK(hipMemcpy(e->p.work, e->h_work.data(), 4 * (size_t)nw, hipMemcpyHostToDevice));
   if (ns > 0) HIPCK(hipMemcpy(e->p.syslist, e->h_syslist.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
-  const int32_t ni = (int32_t)e->h_items.size();
+  const int32_t ni = (int32_t)e->h_items.size(), nfi = (int32_t)e->h_fitems.size();
   if (ni > 0) HIPCK(hipMemcpy(e->p.items, e->h_items.data(), sizeof(int4) * (size_t)ni, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.nitems_far, &n_far_items, 4, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.nitems_big, &n_big_items, 4, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.nwork, &nw, 4, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.nsys, &ns, 4, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.nitems, &n_lds_items, 4, hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(e->p.tilesplit, e->tile_split, sizeof(e->tile_split), hipMemcpyHostToDevice));
+  if (nfi > 0) HIPCK(hipMemcpy(e->p.fitems, e->h_fitems.data(), sizeof(int4) * (size_t)nfi, hipMemcpyHostToDevice));
+  // the counters in one copy: nwork[0] = work entries, [16] batch systems,
+  // [44..46] filter items, [48] / [56] / [60] step items (LDS / far / whole-CU)
+  int32_t cnt[64];
+  HIPCK(hipMemcpy(cnt, e->p.nwork, sizeof(cnt), hipMemcpyDeviceToHost));
+  cnt[0] = nw;
+  cnt[16] = ns;
+  cnt[44] = fp.n_lds;
+  cnt[45] = fp.n_big;
+  cnt[46] = fp.n_far;
+  cnt[48] = sp.n_lds;
+  cnt[56] = sp.n_far;
+  cnt[60] = sp.n_big;
+  HIPCK(hipMemcpy(e->p.nwork, cnt, sizeof(cnt), hipMemcpyHostToDevice));
   e->cur_nwork = nw;
   e->cur_nsys = ns;
-  e->cur_nitems = n_lds_items;
-  e->cur_nitems_far = n_far_items;
-  e->cur_nitems_big = n_big_items;
+  e->cur_nitems = sp.n_lds;
+  e->cur_nitems_far = sp.n_far;
+  e->cur_nitems_big = sp.n_big;
+  e->cur_nfitems = fp.n_lds;
+  e->cur_nfitems_far = fp.n_far;
+  e->cur_nfitems_big = fp.n_big;
   return SOP_OK;
 }
 
@@ -843,6 +890,8 @@ static std::string graph_key(const sop_engine *e, int G, int n_nl, int n_pl) {
   os << G << '|' << n_nl << '|' << n_pl << '|' << e->opts.lanes_per_bead << '|' << grid_tiles(e) << '|'
      << grid_sys(e) << '|' << grid_items(e) << '|' << grid_items_big(e) << '|' << grid_items_far(e) << '|'
      << e->cur_lds << '|' << e->cur_lds_big;
+  const StepGrids fg = filter_grids(e);
+  os << '|' << fg.grid << '|' << fg.grid_big << '|' << fg.grid_far << '|' << fg.lds << '|' << fg.lds_big;
   return os.str();
 }
 
@@ -948,7 +997,9 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     if (!(e->opts.pl_shell > 0.0f) || e->opts.pl_shell > 8.0f) e->opts.pl_shell = 0.0f;
   }
   const int lpb = e->opts.lanes_per_bead;
-  if (lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) {
+  const int stt = e->opts.step_tiles;
+  if ((lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) ||
+      (stt != 0 && stt != 1 && stt != 2 && stt != 4 && stt != 8 && stt != 16)) {
     delete e;
     return SOP_ERR_INVALID;
   }

@@ -85,15 +85,17 @@ struct sop_engine {
 
   // work tables (host staging)
   std::vector<int32_t> h_work, h_syslist;
-  std::vector<int4> h_items;
-  int64_t work_cap = 0, syslist_cap = 0, items_cap = 0;
+  std::vector<int4> h_items, h_fitems;
+  int64_t work_cap = 0, syslist_cap = 0, items_cap = 0, fitems_cap = 0;
   int32_t cur_nwork = 0, cur_nsys = 0, cur_nitems = 0, cur_nitems_big = 0, cur_nitems_far = 0;
+  int32_t cur_nfitems = 0, cur_nfitems_big = 0, cur_nfitems_far = 0;  // shell-filter items
   size_t cur_lds = 0;     // dynamic LDS of the step kernel for the current batch
   size_t cur_lds_big = 0; // dynamic LDS of the whole-CU tier
+  size_t cur_flds = 0, cur_flds_big = 0;  // the same for the shell filter's items
   size_t lds_limit = 0;   // largest dynamic LDS the step kernel may use
   int64_t lds_big_cap = 0; // largest system (beads) of the whole-CU LDS tier; 0: no such tier
   int32_t step_blocks = 0; // step-kernel grid (1024-thread blocks); 0 = 2 per compute unit
-  int32_t tile_split[3] = {1, 1, 1};  // current batch: waves per tile of the LDS, whole-CU and far items
+  int32_t cur_step_tiles = 16; // current batch: tiles per step-kernel item
   bool fixed_grids = false; // serving: launch shapes independent of the batch, so
                             // one captured graph serves every batch composition
 
@@ -140,6 +142,7 @@ int grid_items(const sop_engine *e);
 int grid_items_far(const sop_engine *e);
 int grid_items_big(const sop_engine *e);
 StepGrids step_grids(const sop_engine *e);
+StepGrids filter_grids(const sop_engine *e);
 ListGrids list_grids(const sop_engine *e);
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d);
 } // namespace sopk

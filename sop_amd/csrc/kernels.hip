@@ -1420,12 +1420,12 @@ __device__ __forceinline__ void stage_item(const DevPtrs &p, int z, int w, int k
 
 // Work items of one launch: LDS items (tier 0), whole-CU LDS items (tier 1),
 // or the items that gather from global memory (KLDS = false).
-template <bool KLDS>
+template <bool KLDS, bool FILTER>
 __device__ __forceinline__ void item_range(const DevPtrs &p, int tier, int &w0, int &w1) {
-  const int n0 = *p.nitems, n1 = *p.nitems_big;
+  const int n0 = FILTER ? *p.nfitems : *p.nitems, n1 = FILTER ? *p.nfitems_big : *p.nitems_big;
   if (!KLDS) {
     w0 = n0 + n1;
-    w1 = w0 + *p.nitems_far;
+    w1 = w0 + (FILTER ? *p.nfitems_far : *p.nitems_far);
   } else if (tier == 0) {
     w0 = 0;
     w1 = n0;
@@ -1435,8 +1435,10 @@ __device__ __forceinline__ void item_range(const DevPtrs &p, int tier, int &w0, 
   }
 }
 
-// Fused step over work items {first tile, tiles (<= STEP_WAVES), first staged
-// bead or -1, staged beads}: wave w of the block takes tile first + w.  The
+// Fused step over work items {first tile, tiles | waves per tile << 8, first
+// staged bead or -1, staged beads}: with s waves per tile, waves s k .. s k +
+// s - 1 of the block share tile first + k, each taking LPB / s consecutive
+// rounds (tiles * s <= STEP_WAVES).  The
 // LDS kernel (KLDS) takes the items whose systems fit in LDS and first copies
 // the staged arena beads (whole systems, each from its own coordinate buffer)
 // into dynamic LDS, so every partner gather is an LDS read; the other kernel
@@ -1448,12 +1450,7 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: system data in scalar registers
   int w0, w1;
-  item_range<KLDS>(p, tier, w0, w1);
-  // split tiles: waves ns*k .. ns*k + ns - 1 share tile k of the item, each
-  // taking LPB / ns consecutive rounds
-  const int ns = __builtin_amdgcn_readfirstlane(p.tilesplit[KLDS ? tier : 2]);
-  const int tw = wave / ns;
-  const int r0 = (wave % ns) * (LPB / ns), r1 = r0 + LPB / ns;
+  item_range<KLDS, false>(p, tier, w0, w1);
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.items[w];
     if (it.x < 0) continue;
@@ -1462,7 +1459,12 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
       stage_item(p, it.z, it.w, kstep, s_pos, wave, tid & 63);
       __syncthreads();
     }
-    if (tw >= it.y) continue;
+    // split tiles: waves ns*k .. ns*k + ns - 1 share tile k of the item, each
+    // taking LPB / ns consecutive rounds
+    const int ns = __builtin_amdgcn_readfirstlane(item_split(it.y));
+    const int tw = wave / ns;
+    const int r0 = (wave % ns) * (LPB / ns), r1 = r0 + LPB / ns;
+    if (tw >= item_tiles(it.y)) continue;
     const int t = it.x + tw;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
@@ -1679,8 +1681,8 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   }
 }
 
-// Shell filter over the step kernel's work items (same LDS staging): wave w
-// of the block takes tile first + w of its item.
+// Shell filter over its own work items (the step kernel's LDS staging, one
+// wave per tile): wave w of the block takes tile first + w of its item.
 template <bool KLDS>
 __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier, int stage_off) {
   extern __shared__ float4 s_pos[];
@@ -1693,18 +1695,17 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
                                           wave * (64 / 4) * FILTER_SROW
                                     : nullptr;
   int w0, w1;
-  item_range<KLDS>(p, tier, w0, w1);
+  item_range<KLDS, true>(p, tier, w0, w1);
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
-    const int4 it = p.items[w];
+    const int4 it = p.fitems[w];
     if (it.x < 0) continue;
     if (KLDS) {
       __syncthreads();
       stage_item(p, it.z, it.w, kstep, s_pos, wave, tid & 63);
       __syncthreads();
     }
-    // one wave per tile, split or not (the tile order it computes first must
-    // come from one wave)
-    if (wave >= it.y) continue;
+    // one wave per tile (the tile order it computes first must come from one wave)
+    if (wave >= item_tiles(it.y)) continue;
     const int t = it.x + wave;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
@@ -2003,7 +2004,7 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
-  const StepGrids &sg = lg.sg;
+  const StepGrids &sg = lg.fg;
   const bool fork = fork_wanted(fk, sg);
   if (fork) {
     const hipError_t h = fork_begin(fk, st);

@@ -136,12 +136,15 @@ struct DevPtrs {
   int32_t *ncellwork;   // total home cells of the direct pair-list build
   int32_t *nlbuild;     // 1 if a system of the batch materializes its neighbor list at this rebuild
   int32_t *gsort;       // nonzero if a grid build of this rebuild sorts in global memory (too big for LDS)
-  int32_t *tilesplit;   // waves per tile (1, 2 or 4; each takes that share of the rounds) of LDS, whole-CU, far items
   int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
-  int4 *items;          // step-kernel work: {first tile, tiles, first staged bead or -1, staged beads}
+  int4 *items;          // step-kernel work: {first tile, tiles | waves per tile << 8, first staged
+                        // bead or -1, staged beads}; the waves of a tile split its rounds
   int32_t *nitems;       // items that stage their systems in LDS: items[0 .. *nitems)
   int32_t *nitems_big;   // items staged in a whole-CU LDS block: items[*nitems .. + *nitems_big)
   int32_t *nitems_far;   // items that gather from global memory: the *nitems_far after those
+  int4 *fitems;         // shell-filter work, same form with one wave per tile and STEP_WAVES
+                        // tiles per item: fitems[0 .. *nfitems) LDS, then whole-CU, then far
+  int32_t *nfitems, *nfitems_big, *nfitems_far;
 };
 
 // step kernel: blocks of STEP_WAVES waves, one 64-bead tile per wave; an item
@@ -165,12 +168,16 @@ struct StepFork {
   hipEvent_t fork, join;
 };
 
-// Grids of the list kernels: per tile, per system, and the step kernel's
-// work items (the shell filter shares its LDS staging).
+// Grids of the list kernels: per tile, per system, and the shell filter's
+// work items (staged in LDS like the step kernel's).
 struct ListGrids {
   int grid_tiles, grid_sys;
-  StepGrids sg;
+  StepGrids fg;
 };
+
+// step-kernel item: tiles in the low byte of .y, waves per tile above it
+__host__ __device__ inline int item_tiles(int y) { return y & 0xFF; }
+__host__ __device__ inline int item_split(int y) { return y >> 8; }
 
 // systems whose cell sort fits one block's LDS
 constexpr int SORT_N_MAX = 5120;

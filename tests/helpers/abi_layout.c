@@ -14,7 +14,7 @@ int main(void) {
   printf("sop_system_desc %zu\n", sizeof(sop_system_desc));
   F(sop_system_desc, n_native); F(sop_system_desc, params); F(sop_system_desc, step0); F(sop_system_desc, vel);
   printf("sop_engine_opts %zu\n", sizeof(sop_engine_opts));
-  F(sop_engine_opts, lds_beads_max); F(sop_engine_opts, pool_bytes);
+  F(sop_engine_opts, lds_beads_max); F(sop_engine_opts, pool_bytes); F(sop_engine_opts, step_tiles);
   printf("sop_serve_opts %zu\n", sizeof(sop_serve_opts));
   printf("sop_job_result %zu\n", sizeof(sop_job_result));
   F(sop_job_result, steps); F(sop_job_result, t_submit); F(sop_job_result, energies);

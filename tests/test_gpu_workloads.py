@@ -1,13 +1,13 @@
 """The batched and served workloads at their own shapes, against the oracle.
 
 C3  the 512-replica batch (N log-uniform in [200, 5000], T uniform in
-    [0.5, 0.9], distinct Philox keys) runs 70 steps in one batch, crossing the
-    neighbor-list snapshot at step 50 and pair-list rebuilds from shell rows
-    and from the cell grid.  Eight replicas are then checked against the
+    [0.5, 0.9], distinct Philox keys) runs 200 steps in one batch, crossing
+    neighbor-list snapshots every 50 steps and pair-list rebuilds from shell
+    rows and from the cell grid.  Eight replicas are then checked against the
     oracle at their coordinates: the smallest and the largest, two that share
     one staged LDS range, the one closest below the 5,056-bead LDS limit (C3
     has no replica above it), one whose pair list was rebuilt from the cell
-    grid after the first build, and two more.  Lists bit-exact as sorted sets,
+    grid after the first build, and two more (all at step 200).  Lists bit-exact as sorted sets,
     forces within 1e-5 of the largest force, and, after the whole batch runs
     1,000 more steps, trajectories within 1e-4 A RMSD of the FP64 oracle run
     from the same coordinates and Philox stream.
@@ -98,17 +98,18 @@ def shared_items(sizes, per, lds_cap=LDS_BEADS):
 
 
 def test_c3_batch_sampled_replicas_match_oracle():
+    K0 = 200
     systems = W.c3(512)
     sizes = np.array([s.n for s in systems])
     with S.Engine(0) as eng:
         ids = [eng.add(s) for s in systems]
-        eng.run(ids, 70)
-        assert all(eng.step_of(i) == 70 for i in ids)
+        eng.run(ids, K0)
+        assert all(eng.step_of(i) == K0 for i in ids)
         # the full batch has more tiles than wave slots: one wave per tile, 16 tiles per item
         groups = shared_items(sizes, 16)
         pair = next(g for g in groups if len(g) >= 2)[:2]
         regrid = [r for r in range(len(systems)) if eng.list_stats(ids[r]).grid_builds >= 2]
-        assert regrid, "no replica rebuilt its shell rows from the cell grid within 70 steps"
+        assert regrid, f"no replica rebuilt its shell rows from the cell grid within {K0} steps"
         below = int(np.argmax(np.where(sizes <= LDS_BEADS, sizes, -1)))
         pick = [int(np.argmin(sizes)), int(np.argmax(sizes)), *pair, below, regrid[0]]
         rng = np.random.default_rng(7)
@@ -116,13 +117,13 @@ def test_c3_batch_sampled_replicas_match_oracle():
             pick.append(int(rng.integers(len(systems))))
         pick = sorted(set(pick))
         print("C3 replicas checked:", [(r, int(sizes[r])) for r in pick], "sharing:", pair, flush=True)
-        x70 = {r: eng.coords(ids[r]) for r in pick}
+        x0 = {r: eng.coords(ids[r]) for r in pick}
         for r in pick:
-            check_lists_forces(eng, ids[r], systems[r], x70[r])
+            check_lists_forces(eng, ids[r], systems[r], x0[r])
         eng.run(ids, 1000)
         got = {r: eng.coords(ids[r]) for r in pick}
     with ThreadPoolExecutor(8) as ex:
-        ref = dict(zip(pick, ex.map(lambda r: oracle_final(systems[r], x70[r], 70, 1000), pick)))
+        ref = dict(zip(pick, ex.map(lambda r: oracle_final(systems[r], x0[r], K0, 1000), pick)))
     for r in pick:
         d = rmsd(got[r], ref[r])
         print(f"replica {r} ({sizes[r]} beads): 1000-step rmsd {d:.2e} A", flush=True)
