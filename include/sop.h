@@ -135,7 +135,8 @@ typedef struct sop_engine_opts {
   int32_t step_tiles;     /* 64-bead tiles per work item of the step kernel: 16, 8, 4, 2 or 1;
                              an item of t tiles runs each tile on min(lanes_per_bead, 16 / t)
                              waves (0 = 16 while the batch fills the device, fewer below) */
-  int32_t reserved_opts;  /* 0 */
+  int32_t step_blocks;    /* step-kernel grid in 1024-thread blocks (0 = 2 per compute unit;
+                             sop_autotune picks one) */
 } sop_engine_opts;
 
 /* --- parameters ---------------------------------------------------------- */
@@ -228,7 +229,31 @@ sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *st);
  * systems' coordinates and counters are restored afterwards. */
 sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t reps,
                                 double *avg_ms);
-/* Pick lanes-per-bead and grid size for the listed systems by timing. */
+/* Settings picked by sop_autotune_ex for the listed systems, by timing whole
+ * hipGraph blocks of their own steps (n_nl steps each, list rebuilds
+ * included), one setting at a time.  Systems fall in size buckets (up to
+ * bucket_max_beads[b] beads; the last bucket takes any size) with their own
+ * pair-list shell width and step-kernel item size. */
+typedef struct sop_tune_result {
+  int32_t lanes_per_bead;      /* 4 or 8 */
+  int32_t step_blocks;         /* step-kernel grid in 1024-thread blocks */
+  int32_t lds_beads_max;       /* largest system staged in LDS at two blocks per compute unit */
+  int32_t buckets;             /* size buckets in use (3) */
+  int32_t bucket_max_beads[3];
+  int32_t step_tiles[3];       /* tiles per step-kernel item (0: by batch size) */
+  float pl_shell[3];           /* pair-list shell width, A */
+  int32_t block_steps;         /* steps of one timed block (n_nl) */
+  double block_ms_start;       /* device time of one block under the starting settings */
+  double block_ms_best;        /* the same under the chosen ones */
+  int32_t trials;
+  int32_t reserved;
+} sop_tune_result;
+/* The systems' coordinates, lists and counters are restored after the
+ * trials (a device snapshot of the engine, which needs that much free
+ * memory); the chosen settings stay in the engine, and systems added later
+ * take their size bucket's shell width.  Not while serving. */
+sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_tune_result *out);
+/* sop_autotune_ex, returning the lanes per bead and the step-kernel grid. */
 sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes,
                         int32_t *grid);
 /* Device self test of the wave-level helpers (DPP scan and group sums,

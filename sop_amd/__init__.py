@@ -194,7 +194,8 @@ class Engine:
     """One HIP device's engine (libsop).  Not thread-safe."""
 
     def __init__(self, device=0, lanes_per_bead=4, grid_blocks=0, use_graphs=True, nl_cap_max=0,
-                 pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None, step_tiles=0):
+                 pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None, step_tiles=0,
+                 step_blocks=0):
         self._h = None
         o = _abi.EngineOpts()
         lib().sop_engine_opts_default(C.byref(o))
@@ -213,6 +214,7 @@ class Engine:
         if pl_shell is not None:
             o.pl_shell = pl_shell  # < 0: no shell (every pair-list rebuild scans the cell grid)
         o.step_tiles = step_tiles  # tiles per step-kernel item (0: by batch size)
+        o.step_blocks = step_blocks  # step-kernel grid in blocks (0: 2 per compute unit)
         h = C.c_void_p()
         _check(lib().sop_engine_create(device, C.byref(o), C.byref(h)))
         self._h = h
@@ -330,6 +332,18 @@ class Engine:
         ms = C.c_double()
         self._c(lib().sop_time_step_kernel(self._h, _ptr(a, C.c_int32), len(a), reps, C.byref(ms)))
         return ms.value
+
+    def autotune_ex(self, systems) -> dict:
+        """Settings chosen by timing whole graph blocks of these systems
+        (sop_autotune_ex); the systems' state is left as it was."""
+        a = np.ascontiguousarray(np.atleast_1d(systems), np.int32)
+        r = _abi.TuneResult()
+        self._c(lib().sop_autotune_ex(self._h, _ptr(a, C.c_int32), len(a), C.byref(r)))
+        nb = r.buckets
+        return {"lanes_per_bead": r.lanes_per_bead, "step_blocks": r.step_blocks, "lds_beads_max": r.lds_beads_max,
+                "bucket_max_beads": list(r.bucket_max_beads[:nb]), "step_tiles": list(r.step_tiles[:nb]),
+                "pl_shell": [round(v, 3) for v in r.pl_shell[:nb]], "block_steps": r.block_steps,
+                "block_ms_start": r.block_ms_start, "block_ms_best": r.block_ms_best, "trials": r.trials}
 
     def autotune(self, systems):
         a = np.ascontiguousarray(np.atleast_1d(systems), np.int32)

@@ -64,7 +64,7 @@ class EngineOpts(C.Structure):
         ("nl_cap_max", C.c_int32), ("pl_cap_max", C.c_int32), ("cell_cap_max", C.c_int32),
         ("lanes_per_bead", C.c_int32), ("grid_blocks", C.c_int32), ("use_graphs", C.c_int32),
         ("lds_beads_max", C.c_int32), ("pl_shell", C.c_float), ("pool_bytes", C.c_int64),
-        ("step_tiles", C.c_int32), ("reserved_opts", C.c_int32),
+        ("step_tiles", C.c_int32), ("step_blocks", C.c_int32),
     ]
 
 
@@ -94,6 +94,13 @@ class ServeStats(C.Structure):
                 ("host_batch_s", C.c_double), ("host_block_s", C.c_double), ("host_retire_s", C.c_double),
                 ("mean_running", C.c_double), ("jobs_uploaded_beside", C.c_int64), ("jobs_deferred", C.c_int64),
                 ("bead_steps", C.c_int64)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("lanes_per_bead", C.c_int32), ("step_blocks", C.c_int32), ("lds_beads_max", C.c_int32),
+                ("buckets", C.c_int32), ("bucket_max_beads", C.c_int32 * 3), ("step_tiles", C.c_int32 * 3),
+                ("pl_shell", C.c_float * 3), ("block_steps", C.c_int32), ("block_ms_start", C.c_double),
+                ("block_ms_best", C.c_double), ("trials", C.c_int32), ("reserved", C.c_int32)]
 
 
 class GenOpts(C.Structure):
@@ -138,6 +145,7 @@ SIGNATURES = {
     "sop_list_stats_get": (i32, [vp, i32, P(ListStats)]),
     "sop_time_step_kernel": (i32, [vp, P(i32), i32, i32, P(f64)]),
     "sop_autotune": (i32, [vp, P(i32), i32, P(i32), P(i32)]),
+    "sop_autotune_ex": (i32, [vp, P(i32), i32, P(TuneResult)]),
     "sop_selftest": (i32, [vp, P(i32), P(f32)]),
     "sop_serve_start": (i32, [vp, P(ServeOpts)]),
     "sop_serve_submit": (i32, [vp, P(SystemDesc), i64, P(i64)]),

@@ -271,6 +271,38 @@ static inline int32_t round_up(int32_t v, int32_t a) { return (v + a - 1) / a * 
 
 namespace sopk {
 
+// Pair-list shell of a system: rows within rc_pl + shell of pos_shell, valid
+// while 2 max|x - pos_shell| < shell_ok (the shell less a 1e-4 margin, so the
+// rounding of the shell cutoffs cannot matter); shell 0: no shell rows.  The
+// rows are rebuilt from the cell grid at the next pair-list rebuild.
+void shell_fields(const sop_params &P, float shell, SysDev &S) {
+  float max_rcp = 0.0f;
+  for (int q = 0; q < 4; ++q) {
+    const int ti = q / 2 + 1, tj = q % 2 + 1;
+    max_rcp = std::max(max_rcp, cutoff_host(P.pl_factor, P.pl_abs, P.sigma_rep[ti][tj]));
+  }
+  S.shell_ok = shell * (1.0f - 1.0e-4f);
+  for (int q = 0; q < 4; ++q) {
+    const int ti = q / 2 + 1, tj = q % 2 + 1;
+    const double rs = (double)cutoff_host(P.pl_factor, P.pl_abs, P.sigma_rep[ti][tj]) + (double)shell;
+    S.rc2_sl_rep[q] = (float)(rs * rs);
+  }
+  S.cell_min_edge_sl = std::max(1.0e-3f, (max_rcp + shell) * 1.001f);
+  S.sl_valid = 0;
+}
+
+// Size bucket of a system of n beads in the tuned table (-1: none).
+int engine_bucket(const sop_engine *e, int32_t n) {
+  for (int b = 0; b < e->tune.nb; ++b)
+    if (n <= e->tune.max_beads[b] || b == e->tune.nb - 1) return b;
+  return -1;
+}
+
+float engine_shell_for(const sop_engine *e, int32_t n) {
+  const int b = engine_bucket(e, n);
+  return b >= 0 ? e->tune.shell[b] : e->opts.pl_shell;
+}
+
 sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out) {
   HIPCK(hipMemcpyAsync(out, e->p.sys + slot, sizeof(SysDev), hipMemcpyDeviceToHost, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
@@ -537,12 +569,11 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   S.cell_min_edge = std::max(1.0e-3f, max_rc * 1.001f);
   // pair-list grid and list skin (the direct pair-list build is exact while
   // every bead stays within skin/2 of its neighbor-list snapshot)
-  float max_rcp = 0.0f, skin = INFINITY;
+  float skin = INFINITY;
   for (int q = 0; q < 4; ++q) {
     const int ti = q / 2 + 1, tj = q % 2 + 1;
     const float sg = P.sigma_rep[ti][tj];
     const float rn = cutoff_host(P.nl_factor, P.nl_abs, sg), rp = cutoff_host(P.pl_factor, P.pl_abs, sg);
-    max_rcp = std::max(max_rcp, rp);
     skin = std::min(skin, rn - rp);
   }
   for (int k = 0; k < nn; ++k) {
@@ -550,18 +581,8 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     skin = std::min(skin, cutoff_host(P.nl_factor, P.nl_abs, r0) - cutoff_host(P.pl_factor, P.pl_abs, r0));
   }
   S.skin = skin > 0.0f ? skin * (1.0f - 1.0e-5f) : 0.0f;
-  // pair-list shell: rows within rc_pl + shell of pos_shell, valid while
-  // 2 max|x - pos_shell| < shell_ok (the shell less a 1e-4 margin, so the
-  // rounding of the shell cutoffs cannot matter)
-  const float shell = e->opts.pl_shell;
-  S.shell_ok = shell * (1.0f - 1.0e-4f);
-  for (int q = 0; q < 4; ++q) {
-    const int ti = q / 2 + 1, tj = q % 2 + 1;
-    const double rs = (double)cutoff_host(P.pl_factor, P.pl_abs, P.sigma_rep[ti][tj]) + (double)shell;
-    S.rc2_sl_rep[q] = (float)(rs * rs);
-  }
-  S.cell_min_edge_sl = std::max(1.0e-3f, (max_rcp + shell) * 1.001f);
-  S.sl_valid = 0;
+  h.shell = engine_shell_for(e, n);
+  shell_fields(P, h.shell, S);
   S.build = BUILD_NONE;
   S.nl_valid = 0;
   int tmask = 0;
@@ -685,7 +706,12 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     int32_t n_lds = 0, n_big = 0, n_far = 0;
     size_t lds = 0, lds_big = 0;
   };
-  auto plan = [&](int32_t per, bool split, Plan &out) {
+  // tiles per item of a system: its size bucket's tuned value, else per_step
+  auto per_of = [&](const SysHost &h) {
+    const int b = engine_bucket(e, h.n);
+    return b >= 0 && e->tune.tiles[b] > 0 ? e->tune.tiles[b] : per_step;
+  };
+  auto plan = [&](int32_t per_fixed, bool split, Plan &out) {
     // waves per tile of an item of nt tiles
     auto waves = [&](int32_t nt) {
       int32_t w = 1;
@@ -699,6 +725,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     std::vector<int4> lds_items, big_items;
     std::vector<std::vector<int4>> far_items;  // per non-LDS system
     int4 cur = make_int4(-1, 0, -1, 0);        // .y: tiles while the item grows
+    int32_t cur_per = 0;                       // tiles per item of the systems in cur
     auto flush = [&]() {
       if (cur.x >= 0) lds_items.push_back(mk(cur.x, cur.y, cur.z, cur.w));
       cur = make_int4(-1, 0, -1, 0);
@@ -706,6 +733,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     for (int32_t sl : by_arena) {
       const SysHost &h = e->sys[sl];
       const int32_t t0 = (int32_t)(h.bead_off / TILE), nt = (int32_t)(h.bead_len / TILE);
+      const int32_t per = per_fixed > 0 ? per_fixed : per_of(h);
       if (h.bead_len <= lds_cap) {
         out.lds = std::max(out.lds, (size_t)h.bead_len * 16);
         if (nt > per) {
@@ -714,7 +742,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
             lds_items.push_back(mk(t0 + k, std::min(per, nt - k), (int32_t)h.bead_off, (int32_t)h.bead_len));
           continue;
         }
-        if (cur.x >= 0 && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= per &&
+        if (cur.x >= 0 && cur_per == per && (int64_t)cur.z + cur.w == h.bead_off && cur.y + nt <= per &&
             (int64_t)cur.w + h.bead_len <= lds_cap) {
           cur.y += nt;
           cur.w += (int32_t)h.bead_len;
@@ -722,6 +750,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
         } else {
           flush();
           cur = make_int4(t0, nt, (int32_t)h.bead_off, (int32_t)h.bead_len);
+          cur_per = per;
         }
       } else if (h.bead_len <= big_cap) {
         flush();
@@ -783,7 +812,7 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     if (e->fixed_grids && lds_cap > 0) out.lds = (size_t)lds_cap * 16;
   };
   Plan sp, fp;
-  plan(per_step, true, sp);
+  plan(0, true, sp);
   plan(STEP_WAVES, false, fp);
   e->h_items.swap(sp.items);
   e->h_fitems.swap(fp.items);
@@ -949,6 +978,59 @@ sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps) {
   return SOP_OK;
 }
 
+static void mirror_advance(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
+  for (int q = 0; q < nsys; ++q) {
+    e->sys[sys[q]].step += nsteps;
+    e->sys[sys[q]].parity = (int32_t)((e->sys[sys[q]].parity + nsteps) & 1);
+  }
+}
+
+// nsteps of the listed systems in one batch: hipGraph blocks of n_nl steps
+// while the schedule is uniform and aligned, host-driven launches for the
+// rest.  With events, ev0 is recorded after the batch's tables are set and
+// ev1 after the last launch (device time of the steps).
+sop_status engine_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps, hipEvent_t ev0,
+                      hipEvent_t ev1) {
+  if (nsteps == 0) return SOP_OK;
+  sop_status st = engine_set_batch(e, sys, nsys);
+  if (st) return st;
+  HIPCK(launch_begin(e->p, nsteps, e->stream));
+  if (ev0) HIPCK(hipEventRecord(ev0, e->stream));
+  // graph blocks need a uniform schedule and phase
+  const sop_params &P0 = e->sys[sys[0]].params;
+  bool uniform = e->opts.use_graphs != 0 && P0.n_nl % P0.n_pl == 0;
+  const int64_t G = P0.n_nl;
+  const int64_t r0 = e->sys[sys[0]].step % G;
+  for (int q = 1; q < nsys && uniform; ++q) {
+    const SysHost &h = e->sys[sys[q]];
+    uniform = h.params.n_nl == P0.n_nl && h.params.n_pl == P0.n_pl && h.step % G == r0;
+  }
+  int64_t left = nsteps;
+  if (uniform && G > 1) {
+    const int64_t pre = std::min(left, (G - r0) % G);
+    if (pre > 0) {
+      st = engine_run_hostloop(e, pre);
+      if (st) return st;
+      mirror_advance(e, sys, nsys, pre);
+      left -= pre;
+    }
+    while (left >= G) {
+      st = engine_run_block_graph(e, (int32_t)G);
+      if (st) return st;
+      mirror_advance(e, sys, nsys, G);
+      left -= G;
+    }
+  }
+  if (left > 0) {
+    st = engine_run_hostloop(e, left);
+    if (st) return st;
+    mirror_advance(e, sys, nsys, left);
+  }
+  if (ev1) HIPCK(hipEventRecord(ev1, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  return engine_check_errors(e, sys, nsys);
+}
+
 } // namespace sopk
 
 // ================================================================ C ABI
@@ -1014,6 +1096,7 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     return SOP_ERR_HIP;
   }
   e->num_cu = prop.multiProcessorCount;
+  e->step_blocks = e->opts.step_blocks > 0 ? e->opts.step_blocks : 0;
   // the step kernel keeps 1 KB of static LDS for the tile's forces
   e->lds_limit = prop.sharedMemPerBlock > 2048 ? prop.sharedMemPerBlock - 2048 : 0;
   if (step_lds_limit(e->lds_limit) != hipSuccess) {
@@ -1320,52 +1403,10 @@ static sop_status check_run_args(sop_engine *e, const int32_t *sys, int32_t nsys
   return SOP_OK;
 }
 
-static void mirror_advance(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
-  for (int q = 0; q < nsys; ++q) {
-    e->sys[sys[q]].step += nsteps;
-    e->sys[sys[q]].parity = (int32_t)((e->sys[sys[q]].parity + nsteps) & 1);
-  }
-}
-
 sop_status sop_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
   sop_status st = check_run_args(e, sys, nsys, nsteps);
   if (st) return st;
-  if (nsteps == 0) return SOP_OK;
-  st = engine_set_batch(e, sys, nsys);
-  if (st) return st;
-  HIPCK(launch_begin(e->p, nsteps, e->stream));
-  // graph blocks need a uniform schedule and phase
-  const sop_params &P0 = e->sys[sys[0]].params;
-  bool uniform = e->opts.use_graphs != 0 && P0.n_nl % P0.n_pl == 0;
-  const int64_t G = P0.n_nl;
-  const int64_t r0 = e->sys[sys[0]].step % G;
-  for (int q = 1; q < nsys && uniform; ++q) {
-    const SysHost &h = e->sys[sys[q]];
-    uniform = h.params.n_nl == P0.n_nl && h.params.n_pl == P0.n_pl && h.step % G == r0;
-  }
-  int64_t left = nsteps;
-  if (uniform && G > 1) {
-    const int64_t pre = std::min(left, (G - r0) % G);
-    if (pre > 0) {
-      st = engine_run_hostloop(e, pre);
-      if (st) return st;
-      mirror_advance(e, sys, nsys, pre);
-      left -= pre;
-    }
-    while (left >= G) {
-      st = engine_run_block_graph(e, (int32_t)G);
-      if (st) return st;
-      mirror_advance(e, sys, nsys, G);
-      left -= G;
-    }
-  }
-  if (left > 0) {
-    st = engine_run_hostloop(e, left);
-    if (st) return st;
-    mirror_advance(e, sys, nsys, left);
-  }
-  HIPCK(hipStreamSynchronize(e->stream));
-  return engine_check_errors(e, sys, nsys);
+  return engine_run(e, sys, nsys, nsteps, nullptr, nullptr);
 }
 
 sop_status sop_run_plain(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps) {
@@ -1596,38 +1637,6 @@ sop_status sop_time_step_kernel(sop_engine *e, const int32_t *sys, int32_t nsys,
   HIPCK(hipStreamSynchronize(e->stream));
   HIPCK(hipFree(save));
   *avg_ms = ms / reps;
-  return SOP_OK;
-}
-
-sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes, int32_t *grid) {
-  if (!e) return SOP_ERR_INVALID;
-  const int32_t saved_l = e->opts.lanes_per_bead, saved_g = e->step_blocks;
-  double best = 1e30;
-  int32_t bl = saved_l, bg = saved_g;
-  for (int32_t l : {4, 8, 16}) {
-    for (int32_t gm : {1, 2, 3, 4}) {
-      e->opts.lanes_per_bead = l;
-      e->step_blocks = gm * e->num_cu;
-      double ms = 0.0;
-      sop_status st = sop_time_step_kernel(e, sys, nsys, 3, &ms);  // warm
-      if (st == SOP_OK) st = sop_time_step_kernel(e, sys, nsys, 10, &ms);
-      if (st) {
-        e->opts.lanes_per_bead = saved_l;
-        e->step_blocks = saved_g;
-        return st;
-      }
-      if (ms < best) {
-        best = ms;
-        bl = l;
-        bg = gm * e->num_cu;
-      }
-    }
-  }
-  e->opts.lanes_per_bead = bl;
-  e->step_blocks = bg;
-  invalidate_graphs(e);
-  if (lanes) *lanes = bl;
-  if (grid) *grid = bg;
   return SOP_OK;
 }
 

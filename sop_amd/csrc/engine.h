@@ -55,6 +55,7 @@ struct SysHost {
   int64_t bead_off = 0, nat_off = 0, nl_off = 0, pl_off = 0, cell_off = 0;
   int64_t bead_len = 0, nat_len = 0, nl_len = 0, pl_len = 0, cell_len = 0;
   int32_t nl_cap = 0, pl_cap = 0, cell_cap = 0;
+  float shell = 0.0f; // pair-list shell width (A) of this system
   int64_t step = 0;  // host mirror (exact outside serving)
   int32_t parity = 0;
   sop_params params{};
@@ -99,6 +100,16 @@ struct sop_engine {
   bool fixed_grids = false; // serving: launch shapes independent of the batch, so
                             // one captured graph serves every batch composition
 
+  // settings by system size picked by sop_autotune: systems of up to
+  // max_beads[b] beads (the last bucket: any size) use shell[b] and items of
+  // tiles[b] tiles (0: by batch size); nb = 0: the engine options
+  struct Tune {
+    int32_t nb = 0;
+    int32_t max_beads[3] = {0, 0, 0};
+    float shell[3] = {0, 0, 0};
+    int32_t tiles[3] = {0, 0, 0};
+  } tune;
+
   // graphs keyed by block shape
   std::map<std::string, hipGraphExec_t> graphs;
 
@@ -132,10 +143,15 @@ sop_status engine_remove_system(sop_engine *e, int32_t slot);
 sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots);
 sop_status engine_run_block_graph(sop_engine *e, int32_t G);
 sop_status engine_run_hostloop(sop_engine *e, int64_t nsteps);
+sop_status engine_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t nsteps, hipEvent_t ev0,
+                      hipEvent_t ev1);
 sop_status engine_check_errors(sop_engine *e, const int32_t *slots, int32_t nslots);
 // Status (and error message) of system slot s from its error bits, on the host.
 sop_status engine_error_status(sop_engine *e, int32_t s, int32_t err);
 sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out);
+void shell_fields(const sop_params &P, float shell, SysDev &S);
+int engine_bucket(const sop_engine *e, int32_t n);
+float engine_shell_for(const sop_engine *e, int32_t n);
 int grid_tiles(const sop_engine *e);
 int grid_sys(const sop_engine *e);
 int grid_items(const sop_engine *e);

@@ -20,6 +20,8 @@ int main(void) {
   F(sop_job_result, steps); F(sop_job_result, t_submit); F(sop_job_result, energies);
   printf("sop_list_stats %zu\n", sizeof(sop_list_stats));
   printf("sop_gen_opts %zu\n", sizeof(sop_gen_opts));
+  printf("sop_tune_result %zu\n", sizeof(sop_tune_result));
+  F(sop_tune_result, pl_shell); F(sop_tune_result, block_ms_start); F(sop_tune_result, trials);
   printf("sop_serve_stats %zu\n", sizeof(sop_serve_stats));
   F(sop_serve_stats, gpu_block_s); F(sop_serve_stats, mean_running); F(sop_serve_stats, bead_steps);
   return 0;

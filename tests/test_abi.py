@@ -47,7 +47,7 @@ def test_struct_sizes_match_c_layout(tmp_path):
     structs = {"sop_params": _abi.Params, "sop_system_desc": _abi.SystemDesc, "sop_engine_opts": _abi.EngineOpts,
                "sop_serve_opts": _abi.ServeOpts, "sop_job_result": _abi.JobResult,
                "sop_list_stats": _abi.ListStats, "sop_gen_opts": _abi.GenOpts,
-               "sop_serve_stats": _abi.ServeStats}
+               "sop_serve_stats": _abi.ServeStats, "sop_tune_result": _abi.TuneResult}
     checked = 0
     for line in filter(None, out):
         name, val = line.split()
