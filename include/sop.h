@@ -187,6 +187,16 @@ sop_status sop_force_eval(sop_engine *e, int32_t sys, int32_t with_noise);
 sop_status sop_get_forces(sop_engine *e, int32_t sys, float *f3n);
 /* energy_eval(): e[SOP_E_COUNT] over the current pair list. */
 sop_status sop_energy_eval(sop_engine *e, int32_t sys, double *energies);
+/* sop_energy_eval without any device allocation: no hipMalloc, hipFree or
+ * allocating library call.  No counterpart in the reference; for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid it).  The work table and the energy partials live in a
+ * file-scope device array of the library, ~3 MiB per device, held until the
+ * process ends; systems of up to 65,536 beads (SOP_ERR_CAPACITY above).
+ * Same kernels and summation order as sop_energy_eval: identical results.
+ * Every call on one device shares that device's array: calls that overlap on
+ * one device (engines on the same device driven from different threads) race. */
+sop_status sop_energy_eval_noalloc(sop_engine *e, int32_t sys, double *energies);
 /* calculate_observables() (sop.cu:528-611): Q, native contact count, Rg^2. */
 sop_status sop_observables(sop_engine *e, int32_t sys, double *Q, int32_t *contacts,
                            double *rgsq);

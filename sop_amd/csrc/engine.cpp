@@ -1096,6 +1096,11 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     return SOP_ERR_HIP;
   }
   e->num_cu = prop.multiProcessorCount;
+  // the allocation-free entry point's scratch on this device (the device is current)
+  if (noalloc_scratch(device, &e->scratch) != hipSuccess) {
+    (void)hipGetLastError();
+    e->scratch = nullptr;
+  }
   e->step_blocks = e->opts.step_blocks > 0 ? e->opts.step_blocks : 0;
   // the step kernel keeps 1 KB of static LDS for the tile's forces
   e->lds_limit = prop.sharedMemPerBlock > 2048 ? prop.sharedMemPerBlock - 2048 : 0;
@@ -1376,6 +1381,37 @@ sop_status sop_energy_eval(sop_engine *e, int32_t sys, double *energies) {
   double o[SYSOUT_N];
   sop_status st = eval_energy(e, sys, o);
   if (st) return st;
+  for (int k = 0; k < SOP_E_COUNT; ++k) energies[k] = o[k];
+  return SOP_OK;
+}
+
+sop_status sop_energy_eval_noalloc(sop_engine *e, int32_t sys, double *energies) {
+  if (!valid_sys(e, sys) || !energies) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  if (!e->scratch) return set_error(e, SOP_ERR_HIP, "sop_energy_eval_noalloc: no scratch address on this device");
+  const SysHost &h = e->sys[sys];
+  if (h.n > NOALLOC_BEADS) {
+    std::ostringstream os;
+    os << "sop_energy_eval_noalloc: system " << sys << " has " << h.n << " beads, above the scratch limit of "
+       << NOALLOC_BEADS;
+    return set_error(e, SOP_ERR_CAPACITY, os.str());
+  }
+  // the engine's arrays, with the work table, counters and partials in scratch
+  DevPtrs q = e->p;
+  char *b = static_cast<char *>(e->scratch);
+  q.work = reinterpret_cast<int32_t *>(b + NOALLOC_OFF_WORK);
+  q.nwork = reinterpret_cast<int32_t *>(b + NOALLOC_OFF_CNT);
+  q.nsys = q.nwork + 16;
+  q.syslist = reinterpret_cast<int32_t *>(b + NOALLOC_OFF_SYSLIST);
+  q.epart = reinterpret_cast<double *>(b + NOALLOC_OFF_EPART);
+  q.epart_base = (int32_t)h.bead_off;
+  q.sysout = reinterpret_cast<double *>(b + NOALLOC_OFF_SYSOUT);
+  q.sysout_base = sys;
+  const int t0 = (int)(h.bead_off / TILE), nt = (int)(h.bead_len / TILE);
+  const int grid = std::max(8, std::min(8 * e->num_cu, (nt + 7) / 8 * 8));
+  HIPCK(launch_noalloc_energy(q, t0, nt, sys, grid, e->stream));
+  double o[SYSOUT_N];
+  HIPCK(hipMemcpyAsync(o, q.sysout, sizeof(o), hipMemcpyDeviceToHost, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
   for (int k = 0; k < SOP_E_COUNT; ++k) energies[k] = o[k];
   return SOP_OK;
 }

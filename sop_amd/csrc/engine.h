@@ -27,6 +27,8 @@ hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStre
 hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st);
 hipError_t launch_begin(const DevPtrs &p, int64_t nsteps, hipStream_t st);
 hipError_t launch_selftest(int *out, float *fout, hipStream_t st);
+hipError_t noalloc_scratch(int device, void **ptr);
+hipError_t launch_noalloc_energy(const DevPtrs &p, int t0, int nt, int slot, int grid_tiles, hipStream_t st);
 
 // First-fit range allocator over [0, cap).
 class RangeAlloc {
@@ -72,6 +74,7 @@ struct sop_engine {
   int device = 0;
   hipStream_t stream = nullptr;
   sopk::StepFork fork{nullptr, nullptr, nullptr};  // whole-CU and global-gather launches beside the LDS ones
+  void *scratch = nullptr;  // this device's copy of the allocation-free entry point's scratch
   int num_cu = 0;
   sop_engine_opts opts{};
   std::string err;

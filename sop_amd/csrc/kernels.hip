@@ -20,6 +20,10 @@
 // one launch covers any batch of systems and a captured graph stays valid when
 // the batch changes.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "sop_dev.h"
@@ -1850,7 +1854,7 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
         con += __shfl_xor(con, o);
       }
       if (l == 0 && i < bend) {
-        double *o = p.epart + (int64_t)i * EPART_N;
+        double *o = p.epart + (int64_t)(i - p.epart_base) * EPART_N;
         o[0] = eb; o[1] = ea; o[2] = en; o[3] = er; o[4] = vir; o[5] = con;
       }
     }
@@ -1873,7 +1877,7 @@ __global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs p) {
     for (int k = 0; k < 10; ++k) acc[k] = 0.0;
     for (int b = tid; b < S.n; b += blockDim.x) {
       const int i = S.bead_off + b;
-      const double *e = p.epart + (int64_t)i * EPART_N;
+      const double *e = p.epart + (int64_t)(i - p.epart_base) * EPART_N;
 #pragma unroll
       for (int k = 0; k < 6; ++k) acc[k] += e[k];
       const float4 x = pin[i];
@@ -1890,7 +1894,7 @@ __global__ __launch_bounds__(256) void k_sys_reduce(DevPtrs p) {
     if (tid == 0) {
       double tot[10];
       for (int k = 0; k < 10; ++k) tot[k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
-      double *o = p.sysout + (int64_t)s * SYSOUT_N;
+      double *o = p.sysout + (int64_t)(s - p.sysout_base) * SYSOUT_N;
       o[0] = tot[0]; o[1] = tot[1]; o[2] = tot[2]; o[3] = tot[3];
       o[4] = tot[0] + tot[1] + tot[2] + tot[3];
       o[5] = tot[4];
@@ -2103,6 +2107,57 @@ hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st) {
 
 hipError_t launch_begin(const DevPtrs &p, int64_t nsteps, hipStream_t st) {
   hipLaunchKernelGGL(k_begin, dim3(4), dim3(256), 0, st, p, nsteps);
+  return SOPK_LAUNCH_CHECK();
+}
+
+// ------------------------------------------ allocation-free energy entry point
+//
+// The work table, counters and energy partials of sop_energy_eval_noalloc
+// live in this file-scope array instead of memory the library allocates.
+// Host code never passes the array's name to a kernel or a copy (in host
+// code it stands for no device address): the address of each device's copy
+// comes from hipGetSymbolAddress, once per device, and only that address is
+// used.  Every call on one device shares that device's copy: calls that
+// overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device ordinal -> address of its copy of scratch
+}  // namespace
+
+// The device address of `device`'s copy of scratch; the device must be current.
+hipError_t noalloc_scratch(int device, void **ptr) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void *p = nullptr;
+  const hipError_t h = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (h != hipSuccess) return h;
+  g_scratch_ptr[device] = p;
+  *ptr = p;
+  return hipSuccess;
+}
+
+// Work table of one system (tiles t0 .. t0 + nt - 1) and its counters, in
+// the scratch that p.work / p.nwork / p.nsys / p.syslist point into.
+__global__ void k_noalloc_setup(DevPtrs p, int t0, int nt, int slot) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nt; k += gridDim.x * blockDim.x) p.work[k] = t0 + k;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *p.nwork = nt;
+    *p.nsys = 1;
+    p.syslist[0] = slot;
+  }
+}
+
+hipError_t launch_noalloc_energy(const DevPtrs &p, int t0, int nt, int slot, int grid_tiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(std::max(1, std::min(64, (nt + 255) / 256))), dim3(256), 0, st, p, t0,
+                     nt, slot);
+  hipLaunchKernelGGL(k_energy, dim3(grid_tiles), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_sys_reduce, dim3(1), dim3(256), 0, st, p);
   return SOPK_LAUNCH_CHECK();
 }
 

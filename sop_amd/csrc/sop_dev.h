@@ -145,6 +145,9 @@ struct DevPtrs {
   int4 *fitems;         // shell-filter work, same form with one wave per tile and STEP_WAVES
                         // tiles per item: fitems[0 .. *nfitems) LDS, then whole-CU, then far
   int32_t *nfitems, *nfitems_big, *nfitems_far;
+  // epart and sysout are indexed from these (0 for the engine's arrays; the
+  // first bead and the slot of the one system when they point into scratch)
+  int32_t epart_base, sysout_base;
 };
 
 // step kernel: blocks of STEP_WAVES waves, one 64-bead tile per wave; an item
@@ -185,5 +188,17 @@ constexpr int SORT_C_MAX = 4096;
 
 constexpr int EPART_N = 6;   // bond, angle, native, rep, virial, contacts
 constexpr int SYSOUT_N = 16; // e[0..5], contacts, rgsq, sum x,y,z, n
+
+// Scratch of the allocation-free energy entry point: one system of up to
+// NOALLOC_BEADS beads.  Layout in bytes: work table (one entry per tile),
+// counters (the engine's nwork layout: [0] work entries, [16] systems),
+// system list, per-bead energy partials, the system's reduced values.
+constexpr int NOALLOC_BEADS = 65536;
+constexpr size_t NOALLOC_OFF_WORK = 0;
+constexpr size_t NOALLOC_OFF_CNT = NOALLOC_OFF_WORK + 4 * (NOALLOC_BEADS / TILE) + 256;
+constexpr size_t NOALLOC_OFF_SYSLIST = NOALLOC_OFF_CNT + 256;
+constexpr size_t NOALLOC_OFF_EPART = NOALLOC_OFF_SYSLIST + 256;
+constexpr size_t NOALLOC_OFF_SYSOUT = NOALLOC_OFF_EPART + 8 * EPART_N * (size_t)NOALLOC_BEADS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_OFF_SYSOUT + 8 * SYSOUT_N;
 
 } // namespace sopk
