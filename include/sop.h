@@ -137,6 +137,10 @@ typedef struct sop_engine_opts {
                              waves (0 = 16 while the batch fills the device, fewer below) */
   int32_t step_blocks;    /* step-kernel grid in 1024-thread blocks (0 = 2 per compute unit;
                              sop_autotune picks one) */
+  int32_t step_tail;      /* 2 or 4: step items that start after a first round of full items
+                             are cut to 16 / step_tail tiles with more waves per tile (a
+                             shorter last round); 0 = off */
+  int32_t reserved_opts;  /* 0 */
 } sop_engine_opts;
 
 /* --- parameters ---------------------------------------------------------- */
@@ -256,7 +260,7 @@ typedef struct sop_tune_result {
   double block_ms_start;       /* device time of one block under the starting settings */
   double block_ms_best;        /* the same under the chosen ones */
   int32_t trials;
-  int32_t reserved;
+  int32_t step_tail;           /* as sop_engine_opts.step_tail */
 } sop_tune_result;
 /* The systems' coordinates, lists and counters are restored after the
  * trials (a device snapshot of the engine, which needs that much free

@@ -195,7 +195,7 @@ class Engine:
 
     def __init__(self, device=0, lanes_per_bead=4, grid_blocks=0, use_graphs=True, nl_cap_max=0,
                  pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None, step_tiles=0,
-                 step_blocks=0):
+                 step_blocks=0, step_tail=0):
         self._h = None
         o = _abi.EngineOpts()
         lib().sop_engine_opts_default(C.byref(o))
@@ -215,6 +215,7 @@ class Engine:
             o.pl_shell = pl_shell  # < 0: no shell (every pair-list rebuild scans the cell grid)
         o.step_tiles = step_tiles  # tiles per step-kernel item (0: by batch size)
         o.step_blocks = step_blocks  # step-kernel grid in blocks (0: 2 per compute unit)
+        o.step_tail = step_tail  # 2 or 4: late step items cut finer (0: off)
         h = C.c_void_p()
         _check(lib().sop_engine_create(device, C.byref(o), C.byref(h)))
         self._h = h
@@ -343,7 +344,8 @@ class Engine:
         return {"lanes_per_bead": r.lanes_per_bead, "step_blocks": r.step_blocks, "lds_beads_max": r.lds_beads_max,
                 "bucket_max_beads": list(r.bucket_max_beads[:nb]), "step_tiles": list(r.step_tiles[:nb]),
                 "pl_shell": [round(v, 3) for v in r.pl_shell[:nb]], "block_steps": r.block_steps,
-                "block_ms_start": r.block_ms_start, "block_ms_best": r.block_ms_best, "trials": r.trials}
+                "block_ms_start": r.block_ms_start, "block_ms_best": r.block_ms_best, "trials": r.trials,
+                "step_tail": r.step_tail}
 
     def autotune(self, systems):
         a = np.ascontiguousarray(np.atleast_1d(systems), np.int32)

@@ -65,6 +65,7 @@ class EngineOpts(C.Structure):
         ("lanes_per_bead", C.c_int32), ("grid_blocks", C.c_int32), ("use_graphs", C.c_int32),
         ("lds_beads_max", C.c_int32), ("pl_shell", C.c_float), ("pool_bytes", C.c_int64),
         ("step_tiles", C.c_int32), ("step_blocks", C.c_int32),
+        ("step_tail", C.c_int32), ("reserved_opts", C.c_int32),
     ]
 
 
@@ -100,7 +101,7 @@ class TuneResult(C.Structure):
     _fields_ = [("lanes_per_bead", C.c_int32), ("step_blocks", C.c_int32), ("lds_beads_max", C.c_int32),
                 ("buckets", C.c_int32), ("bucket_max_beads", C.c_int32 * 3), ("step_tiles", C.c_int32 * 3),
                 ("pl_shell", C.c_float * 3), ("block_steps", C.c_int32), ("block_ms_start", C.c_double),
-                ("block_ms_best", C.c_double), ("trials", C.c_int32), ("reserved", C.c_int32)]
+                ("block_ms_best", C.c_double), ("trials", C.c_int32), ("step_tail", C.c_int32)]
 
 
 class GenOpts(C.Structure):

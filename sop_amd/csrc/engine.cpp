@@ -367,6 +367,9 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   // capacities
   const int32_t nl_cap = round_up(std::max(1, std::min(n - 1, e->opts.nl_cap_max)), 16);
   const int32_t pl_cap = round_up(std::max(1, std::min(n - 1, e->opts.pl_cap_max)), 16);
+  // the step kernel addresses a system's pair-list rows with 32-bit byte offsets
+  if ((int64_t)n * pl_cap * (n <= 65536 ? 2 : 4) > (int64_t)0xFFFFFFFF)
+    return set_error(e, SOP_ERR_CAPACITY, "system's pair-list rows exceed 4 GiB (lower pl_cap_max)");
   const int32_t cell_cap = std::min(e->opts.cell_cap_max, std::max(64, 2 * n));
   int32_t dmax = 1;
   while ((int64_t)(dmax + 1) * (dmax + 1) * (dmax + 1) <= cell_cap) ++dmax;
@@ -403,6 +406,10 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   if (st) return st;
   h.nat_off = alloc(e->a_nat, h.nat_len, 1, 1);
   if (st) { e->a_beads.release(h.bead_off, h.bead_len); return st; }
+  if ((h.nat_off + h.nat_len) * 8 > (int64_t)0xFFFFFFFF) {  // 32-bit byte offsets of the native pair list
+    e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
+    return set_error(e, SOP_ERR_CAPACITY, "native pool beyond 4 GiB");
+  }
   h.nl_off = alloc(e->a_nl, h.nl_len, 16, 2);
   if (st) { e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len); return st; }
   h.pl_off = alloc(e->a_pl, h.pl_len, 16, 3);
@@ -672,8 +679,9 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   // staged beads}.  Systems that fit the LDS budget are staged whole:
   // arena-adjacent small systems share an item (one staged range), a larger
   // system gets one item per `per` tiles, each staging the whole system.
-  // Larger systems gather from global memory and keep all their items on one
-  // XCD.  The shell filter takes one wave per tile, STEP_WAVES tiles per item.
+  // Larger systems gather from global memory and keep their items on one XCD
+  // (runs of one XCD's share on several XCDs when they have more items than
+  // that).  The shell filter takes one wave per tile, STEP_WAVES tiles per item.
   // The step kernel's items hold `per` tiles and run each tile on
   // min(lanes_per_bead, STEP_WAVES / tiles) waves (power of two), each wave
   // taking that share of the tile's rounds: an item's duration shrinks with
@@ -775,6 +783,27 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
       return du != dv ? du > dv : item_tiles(u.y) > item_tiles(v.y);
     };
     std::stable_sort(lds_items.begin(), lds_items.end(), by_dur);
+    // the tail: items that start only after a first round of full-length
+    // items (two LDS blocks per compute unit) are cut into pieces of at most
+    // STEP_WAVES / tail tiles, each with more waves per tile, so that the last
+    // round is short
+    if (split && e->tune.tail > 1) {
+      const size_t slots = 2 * (size_t)e->num_cu;
+      size_t nfull = 0;
+      while (nfull < lds_items.size() && item_split(lds_items[nfull].y) == 1) ++nfull;
+      if (nfull > slots) {
+        std::vector<int4> cut(lds_items.begin(), lds_items.begin() + slots);
+        const int32_t pmax = std::max(1, STEP_WAVES / e->tune.tail);
+        for (size_t k = slots; k < lds_items.size(); ++k) {
+          const int4 it = lds_items[k];
+          const int32_t nt = item_tiles(it.y);
+          for (int32_t a = 0; a < nt; a += pmax)
+            cut.push_back(mk(it.x + a, std::min(pmax, nt - a), it.z, it.w));
+        }
+        std::stable_sort(cut.begin() + slots, cut.end(), by_dur);
+        lds_items.swap(cut);
+      }
+    }
     for (const int4 &it : lds_items) {
       const int x = (int)(std::min_element(iload.begin(), iload.end()) - iload.begin());
       ibins[x].push_back(it);
@@ -789,11 +818,18 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     std::sort(far_items.begin(), far_items.end(), [](const std::vector<int4> &u, const std::vector<int4> &v) {
       return u.size() != v.size() ? u.size() > v.size() : u[0].x < v[0].x;
     });
+    // a system with more items than one XCD's share of the global-gather
+    // launch (2 blocks per compute unit) goes out in contiguous runs of that
+    // share, each run to the least-loaded XCD: one XCD would leave the rest of
+    // the device idle, and so large a system outgrows one L2 anyway
+    const size_t share = (size_t)std::max(1, 2 * e->num_cu / NX);
     for (const auto &v : far_items) {
-      const int x = (int)(std::min_element(fload.begin(), fload.end()) - fload.begin());
-      for (const int4 &it : v) {
-        fbins[x].push_back(it);
-        fload[x] += (double)item_tiles(it.y);
+      for (size_t k0 = 0; k0 < v.size(); k0 += share) {
+        const int x = (int)(std::min_element(fload.begin(), fload.end()) - fload.begin());
+        for (size_t k = k0; k < std::min(v.size(), k0 + share); ++k) {
+          fbins[x].push_back(v[k]);
+          fload[x] += (double)item_tiles(v[k].y);
+        }
       }
     }
     size_t imx = 0, bmx = 0, fmx = 0;
@@ -1102,6 +1138,7 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     e->scratch = nullptr;
   }
   e->step_blocks = e->opts.step_blocks > 0 ? e->opts.step_blocks : 0;
+  e->tune.tail = e->opts.step_tail == 2 || e->opts.step_tail == 4 ? e->opts.step_tail : 0;
   // the step kernel keeps 1 KB of static LDS for the tile's forces
   e->lds_limit = prop.sharedMemPerBlock > 2048 ? prop.sharedMemPerBlock - 2048 : 0;
   if (step_lds_limit(e->lds_limit) != hipSuccess) {

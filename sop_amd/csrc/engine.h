@@ -111,6 +111,7 @@ struct sop_engine {
     int32_t max_beads[3] = {0, 0, 0};
     float shell[3] = {0, 0, 0};
     int32_t tiles[3] = {0, 0, 0};
+    int32_t tail = 0;  // > 1: step items after the first round cut to STEP_WAVES / tail tiles
   } tune;
 
   // graphs keyed by block shape

@@ -777,6 +777,26 @@ __device__ __forceinline__ void pad_row(const DevPtrs &p, const SysDev &S, int i
 // which the caller reports as an overflow).
 // The caller's row address and size are wave-uniform; readfirstlane puts
 // the descriptor in scalar registers.
+// Buffer resource of `bytes` bytes at `base` (wave-uniform): loads through
+// it take a 32-bit offset, and a load past the end returns zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
+  const uint64_t a = (uint64_t)base;
+  const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(a >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  return __builtin_amdgcn_make_buffer_rsrc((void *)u, (short)0, (int)__builtin_amdgcn_readfirstlane((int)bytes),
+                                           0x00020000);
+}
+
+__device__ __forceinline__ uint4 load_b128(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ int2 load_i2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  return make_int2((int)v[0], (int)v[1]);
+}
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(void *row, int bytes) {
   const uint64_t a = (uint64_t)row;
   const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(a >> 32)) << 32) |
@@ -1218,7 +1238,13 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   const float br_before = b0 > bead_off ? p.bond_r0[b0 - 1] : 0.0f;  // bond (b0 - 1, b0)
   // the non-native rows in 16-byte chunks: E = 8 16-bit or 4 32-bit entries
   constexpr int E = IDX16 ? 8 : 4;
+  constexpr uint32_t ESZ = IDX16 ? 2u : 4u;
   using chunk_t = uint4;
+  // buffer resources in scalar registers: this system's pair-list rows and
+  // the native pair list, addressed by 32-bit offsets
+  const __amdgpu_buffer_rsrc_t rrow =
+      buf_rsrc(p.plrep_j + S.pl_base, (uint32_t)S.n * (uint32_t)S.pl_cap * ESZ);
+  const __amdgpu_buffer_rsrc_t rnat = buf_rsrc(p.platt, 0xFFFFFFFFu);
   float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;  // force on bead b0 + lane
   for (int r = r0; r < r1; ++r) {
     const int li = __shfl(ordm, r * GPW + g);
@@ -1229,19 +1255,16 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     const float br_low = li > 0 ? br_low_t : br_before;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (i < bend) {
-      // first loads of the native and non-native loops, issued before any gather
-      const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
-      const chunk_t *__restrict__ rowc =
-          IDX16 ? reinterpret_cast<const chunk_t *>((const uint16_t *)(p.plrep_j + S.pl_base) + (pb - S.pl_base))
-                : reinterpret_cast<const chunk_t *>(p.plrep_j + pb);
+      // first loads of the native and non-native loops, issued before any
+      // gather: the lane's first row chunk and first two native entries
+      // ({partner, bits of r0})
+      const uint32_t rowoff = (uint32_t)(i - bead_off) * (uint32_t)S.pl_cap * ESZ;
       const int nch = (nr + E - 1) / E;
       chunk_t q = {};
-      if (l < nch) q = rowc[l];
-      // the lane's first two native entries ({partner, bits of r0}), both
-      // loads issued before any gather
+      if (l < nch) q = load_b128(rrow, rowoff + 16u * (uint32_t)l);
       int2 na = make_int2(0, 0), nb2 = make_int2(0, 0);
-      if (l < nc) na = p.platt[ns + l];
-      if (l + LPB < nc) nb2 = p.platt[ns + l + LPB];
+      if (l < nc) na = load_i2(rnat, 8u * (uint32_t)(ns + l));
+      if (l + LPB < nc) nb2 = load_i2(rnat, 8u * (uint32_t)(ns + l + LPB));
       const float4 xi = partner<LDS>(pin, s_pos, lds0, i);
       const int wi = __float_as_int(xi.w);
       const int row = ONE ? q1 : ((wi & 3) - 1) * 2;
@@ -1282,7 +1305,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           int2 en;
           if (k == l) en = na;
           else if (k == l + LPB) en = nb2;
-          else en = p.platt[ns + k];
+          else en = load_i2(rnat, 8u * (uint32_t)(ns + k));
           const int j = en.x;
           const float r0 = __int_as_float(en.y);
           const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
@@ -1325,7 +1348,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
         };
         for (int c = l; c < nch; c += LPB) {
           const chunk_t cur = q;
-          if (c + LPB < nch) q = rowc[c + LPB];
+          if (c + LPB < nch) q = load_b128(rrow, rowoff + 16u * (uint32_t)(c + LPB));
           uint32_t ww[E];
           if constexpr (IDX16) {
             ww[0] = cur.x & 0xFFFFu; ww[1] = cur.x >> 16; ww[2] = cur.y & 0xFFFFu; ww[3] = cur.y >> 16;

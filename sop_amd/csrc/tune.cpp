@@ -205,6 +205,11 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
       if (w != best.tune.shell[b]) consider(c);
     }
   }
+  for (int32_t t : {0, 2, 4}) {
+    Setting c = best;
+    c.tune.tail = t;
+    if (t != best.tune.tail) consider(c);
+  }
   if (nmax > 3008 && best.lds_beads > 0) {
     for (int32_t l : {5056, 4032, 3008}) {
       Setting c = best;
@@ -238,6 +243,7 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
     out->block_ms_start = base_ms / blocks;
     out->block_ms_best = best_ms / blocks;
     out->trials = trials;
+    out->step_tail = best.tune.tail;
   }
   return SOP_OK;
 }

@@ -154,7 +154,7 @@ def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates
         finally:
             eng.close()
     lat = np.array(lat)
-    offered = sum(n * nst for (_, n, nst) in ids) / seconds
+    offered = sum(n * nst for (_, n, nst) in ids) / seconds if seconds > 0 else 0.0
     out = {"jobs": len(ids), "completed": int(len(lat)), "censored": len(ids) - int(len(lat)),
            "arrival_rate": rate, "arrival_seconds": seconds, "window_s": window,
            "steps_range": [steps_lo, steps_hi], "templates": len(base), "block_steps": block_steps,
@@ -176,5 +176,6 @@ def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates
     rate_s = out.get("bead_steps_per_s") or out.get("capacity_bead_steps_per_s")
     log(f"served: {out['completed']}/{out['jobs']} done ({out['censored']} censored), "
         f"p50 {p50 if p50 is None else round(p50, 3)}s p99 {p99 if p99 is None else round(p99, 3)}s, "
-        f"{rate_s:.4g} bead-steps/s (offered {offered:.4g})" + (f", resident {res_out}" if resident is not None else ""))
+        f"{rate_s if rate_s is None else round(rate_s / 1e9, 3)}e9 bead-steps/s (offered {offered:.4g})"
+        + (f", resident {res_out}" if resident is not None else ""))
     return out
