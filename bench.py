@@ -257,6 +257,7 @@ def main():
                     help="only run a device-to-device copy of BYTES (counter calibration for profiles)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the batched workload's warm-up and steps (the counter probes' child run)")
+    ap.add_argument("--tune", default=None, help="settings (JSON of Engine.autotune_ex) for --traffic-probe")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
@@ -276,6 +277,8 @@ def main():
     log(f"workload: {len(systems)} replicas, {beads} beads, generated in {time.perf_counter() - t:.1f}s")
     if args.traffic_probe:
         with c3_engine(S, args) as eng:
+            if args.tune:
+                eng.tune_set(json.loads(args.tune))
             ids = [eng.add(s) for s in systems]
             eng.run(ids, args.warmup)
             eng.run(ids, args.steps)
@@ -286,9 +289,13 @@ def main():
     ids = [eng.add(s) for s in systems]
     log(f"resident on device in {time.perf_counter() - t:.1f}s")
     lanes, grid, tiles = args.lanes, args.step_blocks, args.step_tiles
+    tuned = tuned_served = None
     if not args.no_autotune:
-        lanes, grid = eng.autotune(ids)
-        log(f"autotune: {lanes} lanes/bead, grid {grid}")
+        t = time.perf_counter()
+        tuned = eng.autotune_ex(ids)
+        lanes, grid = tuned["lanes_per_bead"], tuned["step_blocks"]
+        log(f"autotune ({time.perf_counter() - t:.1f}s, {tuned['trials']} trials of {tuned['block_steps']}-step "
+            f"blocks): {json.dumps(tuned)}")
 
     eng.run(ids, args.warmup)
     log(f"warmup {args.warmup} steps done")
@@ -315,13 +322,24 @@ def main():
 
     from sop_amd import serving
     slanes = args.serve_lanes or lanes
+    if not args.no_autotune:
+        # settings for the served mix: tuned on a resident batch of the stream's job sizes
+        t = time.perf_counter()
+        jobs = W.c4_jobs(rate=200.0, duration=2.0, seed=44)
+        with S.Engine(0, lanes_per_bead=slanes) as et:
+            tids = [et.add(W.c4_system(j, 44)) for j in jobs[:256]]
+            tuned_served = et.autotune_ex(tids)
+        slanes = tuned_served["lanes_per_bead"]
+        log(f"autotune on 256 served-mix jobs ({time.perf_counter() - t:.1f}s): {json.dumps(tuned_served)}")
     served = served_c4 = None
     if args.serve_seconds > 0:
-        served = serving.bench_served(args.serve_seconds, log, lanes_per_bead=slanes, block_steps=args.serve_block)
+        served = serving.bench_served(args.serve_seconds, log, lanes_per_bead=slanes, block_steps=args.serve_block,
+                                      tune=tuned_served)
         served["config"] = "reduced step range [1e3, 1e4] (a load one GPU sustains), drained to the last job"
     if args.c4_window > 0:
         served_c4 = serving.bench_served(args.c4_window, log, steps_lo=1e4, steps_hi=1e6, templates=1024,
-                                         lanes_per_bead=slanes, block_steps=args.serve_block, window=args.c4_window)
+                                         lanes_per_bead=slanes, block_steps=args.serve_block, window=args.c4_window,
+                                         tune=tuned_served)
         served_c4["config"] = ("the stream's stated parameters: 200 jobs/s, steps log-uniform [1e4, 1e6]; offered "
                                "load above capacity, measured over a bounded window")
         served_c4["capacity_vs_batched"] = served_c4["capacity_bead_steps_per_s"] / value
@@ -332,6 +350,8 @@ def main():
         c5 = W.c5()
         log(f"assembly: {c5.n} beads, {len(c5.nat_i)} natives, generated in {time.perf_counter() - t:.1f}s")
         with S.Engine(0, lanes_per_bead=slanes) as e5:
+            if tuned_served:
+                e5.tune_set(tuned_served)
             i5 = e5.add(c5)
             e5.run([i5], 50)
             t0 = time.perf_counter()
@@ -341,10 +361,11 @@ def main():
                  "steps": args.c5_steps, "path": "batched engine, graph blocks"}
         log(f"assembly alone: {alone['ms_per_step']:.3f} ms/step, {alone['bead_steps_per_s']:.4g} bead-steps/s")
         rsteps = 25 * args.c5_steps
-        res_alone = serving.bench_served(0.0, log, lanes_per_bead=slanes, resident=c5, resident_steps=rsteps)
+        res_alone = serving.bench_served(0.0, log, lanes_per_bead=slanes, resident=c5, resident_steps=rsteps,
+                                         tune=tuned_served)
         res_mix = serving.bench_served(min(20.0, args.c4_window or 20.0), log, steps_lo=1e4, steps_hi=1e6,
                                        templates=1024, lanes_per_bead=slanes, window=min(20.0, args.c4_window or 20.0),
-                                       resident=c5, resident_steps=rsteps)
+                                       resident=c5, resident_steps=rsteps, tune=tuned_served)
         assembly = {"n_beads": c5.n, "alone": alone, "served_alone": res_alone["resident"],
                     "served_with_stream": res_mix["resident"],
                     "stream_with_assembly": {k: res_mix[k] for k in ("jobs", "completed", "censored", "p50_latency_s",
@@ -360,6 +381,8 @@ def main():
                  "--lds-beads", str(args.lds_beads), "--pl-cap", str(args.pl_cap)]
         if args.pl_shell is not None:
             child += ["--pl-shell", str(args.pl_shell)]
+        if tuned:
+            child += ["--tune", json.dumps(tuned)]
         traffic = traffic_probe(child, f"void sopk::k_step<{lanes}, 0>")
         if traffic:
             log(f"step kernel HBM traffic {traffic['bytes'] / 1e6:.1f} MB/launch "
@@ -396,6 +419,7 @@ def main():
                         "distinct Philox keys, lists 32/16 A every 50/10 steps, one resident batch",
             "replicas": len(systems), "beads": beads, "lanes_per_bead": lanes, "step_blocks": grid,
             "step_tiles": tiles, "parallelism": "replica batch on 1 GPU",
+            "autotune": tuned, "autotune_served_mix": tuned_served,
         },
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic["bytes"] if traffic else None,

@@ -139,7 +139,7 @@ typedef struct sop_engine_opts {
                              sop_autotune picks one) */
   int32_t step_tail;      /* 2 or 4: step items that start after a first round of full items
                              are cut to 16 / step_tail tiles with more waves per tile (a
-                             shorter last round); 0 = off */
+                             shorter last round); 0 = off (default 2) */
   int32_t reserved_opts;  /* 0 */
 } sop_engine_opts;
 
@@ -267,6 +267,11 @@ typedef struct sop_tune_result {
  * memory); the chosen settings stay in the engine, and systems added later
  * take their size bucket's shell width.  Not while serving. */
 sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_tune_result *out);
+/* Apply settings (from sop_autotune_ex, possibly on another engine with a
+ * representative batch) to this engine: launch shapes, and the size buckets'
+ * shell widths and item sizes for its systems and those added later.  Not
+ * while serving; call it before sop_serve_start to serve with tuned settings. */
+sop_status sop_tune_set(sop_engine *e, const sop_tune_result *r);
 /* sop_autotune_ex, returning the lanes per bead and the step-kernel grid. */
 sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes,
                         int32_t *grid);

@@ -195,7 +195,7 @@ class Engine:
 
     def __init__(self, device=0, lanes_per_bead=4, grid_blocks=0, use_graphs=True, nl_cap_max=0,
                  pl_cap_max=0, cell_cap_max=0, pool_bytes=0, lds_beads_max=0, pl_shell=None, step_tiles=0,
-                 step_blocks=0, step_tail=0):
+                 step_blocks=0, step_tail=None):
         self._h = None
         o = _abi.EngineOpts()
         lib().sop_engine_opts_default(C.byref(o))
@@ -215,7 +215,8 @@ class Engine:
             o.pl_shell = pl_shell  # < 0: no shell (every pair-list rebuild scans the cell grid)
         o.step_tiles = step_tiles  # tiles per step-kernel item (0: by batch size)
         o.step_blocks = step_blocks  # step-kernel grid in blocks (0: 2 per compute unit)
-        o.step_tail = step_tail  # 2 or 4: late step items cut finer (0: off)
+        if step_tail is not None:
+            o.step_tail = step_tail  # 2 or 4: late step items cut finer (0: off; default 2)
         h = C.c_void_p()
         _check(lib().sop_engine_create(device, C.byref(o), C.byref(h)))
         self._h = h
@@ -340,6 +341,24 @@ class Engine:
         a = np.ascontiguousarray(np.atleast_1d(systems), np.int32)
         r = _abi.TuneResult()
         self._c(lib().sop_autotune_ex(self._h, _ptr(a, C.c_int32), len(a), C.byref(r)))
+        return self._tune_dict(r)
+
+    def tune_set(self, tuned: dict):
+        """Apply settings returned by autotune_ex (sop_tune_set)."""
+        r = _abi.TuneResult()
+        r.lanes_per_bead, r.step_blocks, r.lds_beads_max = (tuned["lanes_per_bead"], tuned["step_blocks"],
+                                                            tuned["lds_beads_max"])
+        nb = len(tuned["bucket_max_beads"])
+        r.buckets = nb
+        for b in range(nb):
+            r.bucket_max_beads[b] = tuned["bucket_max_beads"][b]
+            r.step_tiles[b] = tuned["step_tiles"][b]
+            r.pl_shell[b] = tuned["pl_shell"][b]
+        r.step_tail = tuned.get("step_tail", 0)
+        self._c(lib().sop_tune_set(self._h, C.byref(r)))
+
+    @staticmethod
+    def _tune_dict(r) -> dict:
         nb = r.buckets
         return {"lanes_per_bead": r.lanes_per_bead, "step_blocks": r.step_blocks, "lds_beads_max": r.lds_beads_max,
                 "bucket_max_beads": list(r.bucket_max_beads[:nb]), "step_tiles": list(r.step_tiles[:nb]),

@@ -147,6 +147,7 @@ SIGNATURES = {
     "sop_time_step_kernel": (i32, [vp, P(i32), i32, i32, P(f64)]),
     "sop_autotune": (i32, [vp, P(i32), i32, P(i32), P(i32)]),
     "sop_autotune_ex": (i32, [vp, P(i32), i32, P(TuneResult)]),
+    "sop_tune_set": (i32, [vp, P(TuneResult)]),
     "sop_energy_eval_noalloc": (i32, [vp, i32, P(f64)]),
     "sop_selftest": (i32, [vp, P(i32), P(f32)]),
     "sop_serve_start": (i32, [vp, P(ServeOpts)]),

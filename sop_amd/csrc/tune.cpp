@@ -11,6 +11,12 @@
 
 using namespace sopk;
 
+#define HIPCK_TUNE(expr)                                         \
+  do {                                                         \
+    const sop_status _s = hip_check(e, (expr), #expr);           \
+    if (_s != SOP_OK) return _s;                               \
+  } while (0)
+
 namespace {
 
 // Every device array the list and step kernels write, whole, plus the host
@@ -246,6 +252,34 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
     out->step_tail = best.tune.tail;
   }
   return SOP_OK;
+}
+
+sop_status sop_tune_set(sop_engine *e, const sop_tune_result *r) {
+  if (!e || !r) return SOP_ERR_INVALID;
+  if (e->serve) return set_error(e, SOP_ERR_STATE, "engine is serving");
+  if ((r->lanes_per_bead != 4 && r->lanes_per_bead != 8 && r->lanes_per_bead != 16 && r->lanes_per_bead != 32) ||
+      r->buckets < 0 || r->buckets > 3 || r->step_blocks < 0 ||
+      (r->step_tail != 0 && r->step_tail != 2 && r->step_tail != 4))
+    return set_error(e, SOP_ERR_INVALID, "tune: bad settings");
+  Setting st;
+  st.lanes = r->lanes_per_bead;
+  st.blocks = r->step_blocks;
+  st.lds_beads = r->lds_beads_max;
+  st.tune.nb = r->buckets;
+  for (int b = 0; b < r->buckets; ++b) {
+    const int32_t t = r->step_tiles[b];
+    if (t != 0 && t != 1 && t != 2 && t != 4 && t != 8 && t != 16) return set_error(e, SOP_ERR_INVALID, "tune: step_tiles");
+    if (!(r->pl_shell[b] >= 0.0f) || r->pl_shell[b] > 8.0f) return set_error(e, SOP_ERR_INVALID, "tune: pl_shell");
+    st.tune.max_beads[b] = r->bucket_max_beads[b];
+    st.tune.tiles[b] = t;
+    st.tune.shell[b] = r->pl_shell[b];
+  }
+  st.tune.tail = r->step_tail;
+  HIPCK_TUNE(hipStreamSynchronize(e->stream));
+  std::vector<int32_t> all;
+  for (size_t s = 0; s < e->sys.size(); ++s)
+    if (e->sys[s].used) all.push_back((int32_t)s);
+  return apply(e, all.data(), (int32_t)all.size(), st);
 }
 
 sop_status sop_autotune(sop_engine *e, const int32_t *sys, int32_t nsys, int32_t *lanes, int32_t *grid) {

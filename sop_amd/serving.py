@@ -67,7 +67,7 @@ class Server:
 
 def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates=256, seed=4,
                  budget_gib=32.0, drain_timeout=120.0, lanes_per_bead=4, block_steps=0, window=None,
-                 resident=None, resident_steps=0):
+                 resident=None, resident_steps=0, tune=None):
     """Served workload: Poisson arrivals at `rate` jobs/s for `seconds`; job
     sizes log-normal (median 1500, sigma 0.9) clipped to [100, 20000]; steps
     log-uniform in [steps_lo, steps_hi]; T uniform in [0.5, 0.9]; fresh Philox
@@ -86,13 +86,16 @@ def bench_served(seconds, log, rate=200.0, steps_lo=1e3, steps_hi=1e4, templates
 
     resident: a System submitted before the stream with resident_steps steps
     (a large assembly run alongside the served stream); its steps/s from
-    admission to completion is reported."""
+    admission to completion is reported.  tune: settings of Engine.autotune_ex
+    applied to the serving engine before it starts."""
     jobs = W.c4_jobs(rate=rate, duration=seconds, seed=seed, steps_lo=steps_lo, steps_hi=steps_hi)
     t0 = time.perf_counter()
     base = [W.c4_system(jobs[k], seed) for k in range(min(templates, len(jobs)))]
     log(f"served: {len(jobs)} arrivals over {seconds}s, steps in [{steps_lo:.0f}, {steps_hi:.0f}], "
         f"{len(base)} chain templates in {time.perf_counter() - t0:.1f}s")
     eng = Engine(0, lanes_per_bead=lanes_per_bead)
+    if tune:
+        eng.tune_set(tune)
     srv = None
     res_out = None
     try:

@@ -300,6 +300,32 @@ def test_timing_and_autotune_entry_points():
         e.run(ids, 20)
 
 
+def test_autotune_leaves_no_trace_and_settings_transfer():
+    # the trials run from a device snapshot: coordinates, lists and counters
+    # are as before; the chosen settings, applied with sop_tune_set at the
+    # same point of the same run on a fresh engine, give the same trajectories
+    # bitwise
+    systems = batch_systems()
+    with S.Engine(0) as a:
+        ids = [a.add(s) for s in systems]
+        a.run(ids, 30)
+        before = [a.coords(i) for i in ids]
+        tuned = a.autotune_ex(ids)
+        print("tuned:", tuned, flush=True)
+        assert tuned["trials"] > 5 and tuned["block_ms_best"] <= tuned["block_ms_start"] * 1.0001
+        assert all(np.array_equal(a.coords(i), x) for i, x in zip(ids, before))
+        assert all(a.step_of(i) == 30 for i in ids)
+        a.run(ids, 70)
+        after = [a.coords(i) for i in ids]
+    with S.Engine(0) as b:
+        ids = [b.add(s) for s in systems]
+        b.run(ids, 30)
+        b.tune_set(tuned)
+        b.run(ids, 70)
+        for i, x in zip(ids, after):
+            assert np.array_equal(b.coords(i), x)
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 4])
 def test_tiny_chains_match_oracle(eng, n):
     # one bead (no terms at all), one bond, one i,i+2 term, one possible native:
