@@ -141,7 +141,8 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.cell_of, 4},    {(void **)&p.cell_tmp, 4},    {(void **)&p.cell_sorted, 4},
       {(void **)&p.cell_pos, 4},   {(void **)&p.nat_sorted, 8},
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
-      {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16}};
+      {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16},
+      {(void **)&p.plrep_off, 4}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},    {(void **)&p.nat_r0, 4},  {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt, 8}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
@@ -363,6 +364,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     std::sort(nat[i].begin(), nat[i].end());
     for (size_t q = 1; q < nat[i].size(); ++q)
       if (nat[i][q].first == nat[i][q - 1].first) return set_error(e, SOP_ERR_INVALID, "duplicate native pair");
+    if (nat[i].size() > 65535) return set_error(e, SOP_ERR_CAPACITY, "more than 65535 native partners of one bead");
   }
   // capacities
   const int32_t nl_cap = round_up(std::max(1, std::min(n - 1, e->opts.nl_cap_max)), 16);
@@ -391,7 +393,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   // with an upload stream, every upload goes through the pinned staging
   // buffer (a copy from pageable memory would wait for the engine stream)
   const size_t bl = (size_t)h.bead_len, nl_b = (size_t)h.nat_len;
-  const size_t stage_need = 16 * 256 + 16 * bl * 3 + 4 * bl * 7 + 8 * nl_b + bl + 4 * (bl / TILE) + sizeof(SysDev);
+  const size_t stage_need = 32 * 256 + 16 * bl * 3 + 4 * bl * 8 + 8 * nl_b + bl + 4 * (bl / TILE) + sizeof(SysDev);
   if (up && stage_need > upl->stage_bytes) return SOP_ERR_BUSY;
   sop_status st;
   // with an upload stream, free ranges only: growing a pool reallocates
@@ -494,7 +496,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   HIPCK(put(p.bond_r0 + b0, bond.data(), 4 * h.bead_len));
   HIPCK(put(p.nat_start + b0, nstart.data(), 4 * h.bead_len));
   HIPCK(put(p.nat_cnt + b0, ncnt.data(), 4 * h.bead_len));
-  for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt})
+  for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt, p.plrep_off})
     HIPCK(put(arr + b0, zeros.data(), 4 * h.bead_len));
   // underdamped state: the velocities entering the first step (w = 0)
   if (d->vel)
@@ -1117,8 +1119,10 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
   }
   const int lpb = e->opts.lanes_per_bead;
   const int stt = e->opts.step_tiles;
+  // the step kernel holds a bead's pair-list and native pair-list lengths in
+  // 16 bits each: pl_cap_max <= 65535 (natives per bead are checked at add)
   if ((lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) ||
-      (stt != 0 && stt != 1 && stt != 2 && stt != 4 && stt != 8 && stt != 16)) {
+      (stt != 0 && stt != 1 && stt != 2 && stt != 4 && stt != 8 && stt != 16) || e->opts.pl_cap_max > 65535) {
     delete e;
     return SOP_ERR_INVALID;
   }
@@ -1320,7 +1324,7 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
     seg_len = nl ? h.nl_len : h.pl_len;
     ent.resize((size_t)seg_len);
     if (!nl && h.n <= 65536) {
-      // 16-bit local indices, rows of pl_cap entries packed from the segment start
+      // 16-bit local indices from the segment start
       std::vector<uint16_t> e16((size_t)seg_len);
       HIPCK(hipMemcpy(e16.data(), e->p.plrep_j + h.pl_off, 2 * (size_t)seg_len, hipMemcpyDeviceToHost));
       for (int64_t q = 0; q < seg_len; ++q) ent[(size_t)q] = (int32_t)h.bead_off + e16[(size_t)q];
@@ -1328,8 +1332,11 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
       HIPCK(hipMemcpy(ent.data(), (nl ? e->p.nlrep_j + h.nl_off : e->p.plrep_j + h.pl_off), 4 * (size_t)seg_len,
                       hipMemcpyDeviceToHost));
     }
-    const int32_t capb = nl ? h.nl_cap : h.pl_cap;
-    for (int i = 0; i < nb; ++i) start[i] = i * capb;
+    if (nl) {
+      for (int i = 0; i < nb; ++i) start[i] = i * h.nl_cap;
+    } else {  // pair-list rows are packed per tile: each bead's row start is stored
+      HIPCK(hipMemcpy(start.data(), e->p.plrep_off + h.bead_off, 4 * (size_t)nb, hipMemcpyDeviceToHost));
+    }
   } else {
     return set_error(e, SOP_ERR_INVALID, "unknown list");
   }

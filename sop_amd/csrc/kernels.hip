@@ -286,6 +286,7 @@ __global__ __launch_bounds__(SKIN_THREADS) void k_skin(DevPtrs p, int kstep, int
       }
       S.build = build;
       if (build == BUILD_DIRECT) S.sl_valid = 1;
+      if (build == BUILD_NL) S.sl_valid = 0;  // k_pl_build writes its rows into the shell-row slots
       if (build == BUILD_DIRECT || build == BUILD_SHELL || build == BUILD_NL)
         ++S.nbuild[build == BUILD_DIRECT ? 0 : (build == BUILD_SHELL ? 1 : 2)];
       s_build = build;
@@ -752,31 +753,17 @@ constexpr int HOME_MAX = 64;   // home beads per pass
 constexpr int CAND_MAX = 1024; // candidates staged per pass (and bits of the exclusion bitmaps)
 constexpr int EXCL_WORDS = CAND_MAX / 32;
 
-// Pad a pair-list row of cnt entries with the bead's own index up to a
-// multiple of ROW_PAD, so the step kernel reads whole chunks without a count
+// Pair-list rows are padded with the bead's own index up to a multiple of
+// ROW_PAD entries, so the step kernel reads whole chunks without a count
 // test: a self entry has d2 = 0, which the repulsion guard turns into zero
-// force.  pl_cap is a multiple of 16, so the padding stays inside the row.
+// force.  pl_cap is a multiple of 16, so a padded row fits its capacity.
 constexpr int ROW_PAD = 8;
 // LDS of one workgroup on gfx950, and the shell filter's row staging: per
-// wave, 16 beads of a round x FILTER_SROW 16-bit entries (longer rows store
-// the rest directly)
+// wave, 16 beads of a round x FILTER_SROW 16-bit entries (32-bit entries take
+// twice the bytes; longer rows are completed by a second pass)
 constexpr int LDS_BYTES_MAX = 160 * 1024;
 constexpr int FILTER_SROW = 128;
 constexpr int FILTER_STAGE_BYTES = STEP_WAVES * 16 * FILTER_SROW * 2;
-__device__ __forceinline__ void pad_row(const DevPtrs &p, const SysDev &S, int i, int cnt, int k) {
-  const int e = cnt + k;
-  if (k < ((ROW_PAD - (cnt & (ROW_PAD - 1))) & (ROW_PAD - 1)) && e < S.pl_cap) {
-    const int64_t row = (int64_t)(i - S.bead_off) * S.pl_cap;
-    if (S.idx16) ((uint16_t *)(p.plrep_j + S.pl_base))[row + e] = (uint16_t)(i - S.bead_off);
-    else p.plrep_j[S.pl_base + row + e] = i;
-  }
-}
-
-// Output row of one bead as a raw buffer of pl_cap entries: stores past the
-// row's end are dropped by the buffer bounds check (the count still grows,
-// which the caller reports as an overflow).
-// The caller's row address and size are wave-uniform; readfirstlane puts
-// the descriptor in scalar registers.
 // Buffer resource of `bytes` bytes at `base` (wave-uniform): loads through
 // it take a 32-bit offset, and a load past the end returns zeros.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
@@ -797,6 +784,10 @@ __device__ __forceinline__ int2 load_i2(__amdgpu_buffer_rsrc_t r, uint32_t off) 
   return make_int2((int)v[0], (int)v[1]);
 }
 
+// Output row of one bead as a raw buffer of pl_cap entries: stores past the
+// row's end are dropped by the buffer bounds check (the count still grows,
+// which the caller reports as an overflow).  The caller's row address and
+// size are wave-uniform; readfirstlane puts the descriptor in scalar registers.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(void *row, int bytes) {
   const uint64_t a = (uint64_t)row;
   const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(a >> 32)) << 32) |
@@ -1087,7 +1078,10 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
   }
 }
 
-// Wave per bead: pair list = materialized neighbor list filtered at x.
+// Wave per bead: pair list = materialized neighbor list filtered at x.  The
+// native pair list is written in place; the non-native rows go to the
+// shell-row slots (the shell filter packs them into the pair list), which
+// therefore stop holding shell rows (k_skin clears sl_valid).
 __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
   if (*p.nlbuild == 0) return;  // no system of the batch needs it (k_cellwork)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1129,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
       }
       if (lane == 0) p.platt_cnt[i] = cnt;
       const int64_t nb = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
-      const int64_t pb = S.pl_base + (int64_t)(i - S.bead_off) * S.pl_cap;
+      const int64_t pb = S.pl_base + (int64_t)(i - S.bead_off) * S.pl_cap;  // the bead's shell-row slot
       const int nr = p.nlrep_cnt[i];
       const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
       int cr = 0;
@@ -1146,17 +1140,16 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
         if (pred) {
           const int o = cr + lanes_below(mk);
           if (o < S.pl_cap) {
-            if (S.idx16) ((uint16_t *)(p.plrep_j + S.pl_base))[pb - S.pl_base + o] = (uint16_t)(j - S.bead_off);
-            else p.plrep_j[pb + o] = j;
+            if (S.idx16) ((uint16_t *)(p.slrep_j + S.pl_base))[pb - S.pl_base + o] = (uint16_t)(j - S.bead_off);
+            else p.slrep_j[pb + o] = j;
           }
         }
         cr += __popcll(mk);
       }
       if (lane == 0) {
-        p.plrep_cnt[i] = min(cr, S.pl_cap);
+        p.slrep_cnt[i] = min(cr, S.pl_cap);
         if (cr > S.pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
       }
-      pad_row(p, S, i, min(cr, S.pl_cap), lane);
     }
   }
 }
@@ -1227,12 +1220,14 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   const int src = (qm % GPW) * LPB, rm = qm / GPW;
   // per-bead list heads of the whole tile in one coalesced load each; a
   // round picks its beads' values up by shuffles
-  int t_ns = 0, t_nc = 0, t_nr = 0;
+  // (native pair-list length | pair-list length << 16: both below 2^16, which
+  // the host checks)
+  int t_ns = 0, t_cn = 0, t_po = 0;
   float t_br = 0.0f;  // r0 of bond (b0 + lane, b0 + lane + 1)
   if (b0 + lane < bend) {
     t_ns = p.nat_start[b0 + lane];
-    t_nc = p.platt_cnt[b0 + lane];
-    t_nr = p.plrep_cnt[b0 + lane];
+    t_cn = p.platt_cnt[b0 + lane] | (p.plrep_cnt[b0 + lane] << 16);
+    t_po = p.plrep_off[b0 + lane];
     t_br = p.bond_r0[b0 + lane];
   }
   const float br_before = b0 > bead_off ? p.bond_r0[b0 - 1] : 0.0f;  // bond (b0 - 1, b0)
@@ -1240,8 +1235,9 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   constexpr int E = IDX16 ? 8 : 4;
   constexpr uint32_t ESZ = IDX16 ? 2u : 4u;
   using chunk_t = uint4;
-  // buffer resources in scalar registers: this system's pair-list rows and
-  // the native pair list, addressed by 32-bit offsets
+  // buffer resources in scalar registers: this system's pair-list segment
+  // (rows packed per tile, plrep_off) and the native pair list, addressed by
+  // 32-bit offsets
   const __amdgpu_buffer_rsrc_t rrow =
       buf_rsrc(p.plrep_j + S.pl_base, (uint32_t)S.n * (uint32_t)S.pl_cap * ESZ);
   const __amdgpu_buffer_rsrc_t rnat = buf_rsrc(p.platt, 0xFFFFFFFFu);
@@ -1249,7 +1245,8 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   for (int r = r0; r < r1; ++r) {
     const int li = __shfl(ordm, r * GPW + g);
     const int i = b0 + li;
-    const int ns = __shfl(t_ns, li), nc = __shfl(t_nc, li), nr = __shfl(t_nr, li);
+    const int ns = __shfl(t_ns, li), cn = __shfl(t_cn, li), po = __shfl(t_po, li);
+    const int nc = cn & 0xFFFF, nr = cn >> 16;
     // bond r0 of (i, i + 1) and (i - 1, i), from the tile's values
     const float br_up = __shfl(t_br, li), br_low_t = __shfl(t_br, (li + 63) & 63);
     const float br_low = li > 0 ? br_low_t : br_before;
@@ -1258,7 +1255,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       // first loads of the native and non-native loops, issued before any
       // gather: the lane's first row chunk and first two native entries
       // ({partner, bits of r0})
-      const uint32_t rowoff = (uint32_t)(i - bead_off) * (uint32_t)S.pl_cap * ESZ;
+      const uint32_t rowoff = (uint32_t)po * ESZ;
       const int nch = (nr + E - 1) / E;
       chunk_t q = {};
       if (l < nch) q = load_b128(rrow, rowoff + 16u * (uint32_t)l);
@@ -1518,14 +1515,22 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
 // Pair list of one tile's beads = their shell rows filtered at x with the
 // reference's exact predicate (d2 < rc_pl^2, squares summed in order without
 // contraction), for grid and shell builds, and the native pair list from the
-// native CSR.  4 lanes per bead, 16 beads per round: lane l takes 4-entry
-// chunks l, l + 4, ... of the row, kept entries are compacted per chunk
-// position, so a row comes out in (chunk pass, position, lane) order; rows are
-// padded for the step kernel.
+// native CSR.  Neighbor-list builds: k_pl_build has left the filtered rows in
+// the shell-row slots and written the native pair list; they are packed
+// here as they are.  4 lanes per bead, 16 beads per round: lane l
+// takes 4-entry chunks l, l + 4, ... of the row, kept entries are compacted
+// per chunk position, so a row comes out in (chunk pass, position, lane)
+// order.  The rows are packed per tile (plrep_off): a round's row starts
+// follow from its kept counts, so a round first stages up to FILTER_SROW
+// entries per row in LDS (STAGE) and writes them out once the starts are
+// known; the entries past FILTER_SROW, and every entry without staging, come
+// from a second pass over the round's rows, which repeats the first's
+// compaction order.  Rows are padded to ROW_PAD entries for the step kernel.
 template <bool LDS, bool IDX16, bool STAGE>
 __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, int s, int t,
                                             const float4 *__restrict__ pin, const float4 *s_pos, int lds0,
-                                            uint16_t *s_rows) {
+                                            void *s_stage) {
+  using ent_t = typename std::conditional<IDX16, uint16_t, uint32_t>::type;
   using chunk_t = typename std::conditional<IDX16, uint2, uint4>::type;
   const int lane = threadIdx.x & 63, g = lane >> 2, l = lane & 3;
   const uint32_t below = (1u << l) - 1u;
@@ -1534,6 +1539,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   const int bo = S.bead_off, pl_cap = S.pl_cap;
   const int64_t pl_base = S.pl_base;
   const int bend = min(t * TILE + TILE, bo + S.n);
+  const bool nlb = S.build == BUILD_NL;  // neighbor-list build: rows already filtered (k_pl_build)
   // cutoffs as wave-uniform values (a per-lane indexed load would be a vector
   // load, and waiting for it would also wait for the earlier stores)
   const float rc11 = S.rc2_pl_rep[0], rc12 = S.rc2_pl_rep[1], rc21 = S.rc2_pl_rep[2], rc22 = S.rc2_pl_rep[3];
@@ -1544,9 +1550,12 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   const int t_ns = tin ? p.nat_start[t * TILE + lane] : 0, t_nn = tin ? p.nat_cnt[t * TILE + lane] : 0;
   // the tile's bead order for this filter and the step kernel (k_tile_order's
   // key: shell-row length + the native pair-list length of the last rebuild,
-  // decreasing, ties by index), read before this pass rewrites platt_cnt
+  // decreasing, ties by index), read before this pass rewrites platt_cnt;
+  // neighbor-list builds keep the order k_tile_order gave them
   int ordm;
-  {
+  if (nlb) {
+    ordm = p.tile_ord[t * TILE + lane];
+  } else {
     const int key = tin ? t_nr + p.platt_cnt[t * TILE + lane] : -1;
     int rank = 0;
     for (int j = 0; j < TILE; ++j) {
@@ -1558,6 +1567,10 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   }
   const double plf = S.pl_factor;
   const float pla = S.pl_abs;
+  ent_t *const seg = reinterpret_cast<ent_t *>(p.plrep_j + pl_base);
+  // this tile's region of the segment, and the rows packed so far
+  const int region = (t * TILE - bo) * pl_cap;
+  int packed = 0;
   for (int r = 0; r < 4; ++r) {
     const int li = __shfl(ordm, r * 16 + g);
     const int i = t * TILE + li;
@@ -1567,7 +1580,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     // shuffles outside any condition: a lane that a condition turns off does
     // not provide its value to the lanes reading from it
     const int ns = __shfl(t_ns, li), nn_all = __shfl(t_nn, li);
-    const int nn = act ? nn_all : 0;
+    const int nn = act && !nlb ? nn_all : 0;
     // the bead's first NB natives per lane, loaded with the row's first batch
     constexpr int NB = 4;
     int nj[NB];
@@ -1582,65 +1595,70 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
         nr0[b] = p.nat_r0[ns + k];
       }
     }
-    const int nch = (nr + 3) >> 2;
     const float4 xi = partner<LDS>(pin, s_pos, lds0, self);
     const bool t2 = (__float_as_int(xi.w) & 3) == 2;
     const float rcA = t2 ? rc21 : rc11, rcB = t2 ? rc22 : rc12;
-    const int64_t row = (int64_t)(self - bo) * pl_cap;  // entries from the segment start
-    const chunk_t *src = IDX16 ? reinterpret_cast<const chunk_t *>((const uint16_t *)(p.slrep_j + pl_base) + row)
-                               : reinterpret_cast<const chunk_t *>(p.slrep_j + pl_base + row);
-    uint16_t *dst16 = (uint16_t *)(p.plrep_j + pl_base) + row;
-    int32_t *dst32 = p.plrep_j + pl_base + row;
-    uint16_t *s_row = STAGE ? s_rows + g * FILTER_SROW : nullptr;  // this group's staged row
-    // chunks in batches of CB per lane, all loads of a batch issued before
-    // its stores (a load issued after a store waits for the store as well)
-    constexpr int CB = 8;
-    const int clast = max(nch - 1, 0);
-    int cnt = 0;
-    for (int m0 = 0; __ballot(l + 4 * m0 < nch) != 0; m0 += CB) {
-      chunk_t ch[CB];
+    const chunk_t *src = reinterpret_cast<const chunk_t *>(
+        reinterpret_cast<const ent_t *>(p.slrep_j + pl_base) + (int64_t)(self - bo) * pl_cap);
+    ent_t *s_row = STAGE ? reinterpret_cast<ent_t *>(s_stage) + g * FILTER_SROW : nullptr;  // this group's staged row
+    // One pass over the first m entries of the row (m = 0: the group takes
+    // no part): kept entries in (chunk batch, position, lane) order, put(o, j)
+    // for the o-th.  Chunks in batches of CB per lane, all loads of a batch
+    // issued before its stores (a load issued after a store waits for the
+    // store as well).  The order does not depend on CB.
+    auto pass = [&](int m, auto put) -> int {
+      constexpr int CB = IDX16 ? 8 : 4;  // 64 bytes of row loads in flight per lane
+      const int nch = (m + 3) >> 2;
+      const int clast = max(nch - 1, 0);
+      int cnt = 0;
+      for (int m0 = 0; __ballot(l + 4 * m0 < nch) != 0; m0 += CB) {
+        chunk_t ch[CB];
 #pragma unroll
-      for (int b = 0; b < CB; ++b) ch[b] = src[min(l + 4 * (m0 + b), clast)];
+        for (int b = 0; b < CB; ++b) ch[b] = src[min(l + 4 * (m0 + b), clast)];
 #pragma unroll
-      for (int b = 0; b < CB; ++b) {
-        const int c = l + 4 * (m0 + b);
-        if (__ballot(c < nch) == 0) break;
-        int jj[4];
-        if constexpr (IDX16) {
-          jj[0] = bo + (int)(ch[b].x & 0xFFFFu); jj[1] = bo + (int)(ch[b].x >> 16);
-          jj[2] = bo + (int)(ch[b].y & 0xFFFFu); jj[3] = bo + (int)(ch[b].y >> 16);
-        } else {
-          jj[0] = (int)ch[b].x; jj[1] = (int)ch[b].y; jj[2] = (int)ch[b].z; jj[3] = (int)ch[b].w;
-        }
-        bool ok[4];
-        float4 xj[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          ok[e] = 4 * c + e < nr;
-          xj[e] = partner<LDS>(pin, s_pos, lds0, ok[e] ? jj[e] : self);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          // the reference's predicate without the minimum image, which is the
-          // identity here (a grid build whose extent reaches boxl/2 - margin
-          // raises ERR_PBC, and beads move less than the margin before the next)
-          const float dx = xj[e].x - xi.x, dy = xj[e].y - xi.y, dz = xj[e].z - xi.z;
-          const bool keep = ok[e] & (((dx * dx + dy * dy) + dz * dz) < (w_type(xj[e].w) == 2 ? rcB : rcA));
-          const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
-          if (keep) {
-            const int o = cnt + __popc(bits & below);
-            if (STAGE && o < FILTER_SROW) s_row[o] = (uint16_t)(jj[e] - bo);
-            else if (o < pl_cap) {
-              if (IDX16) dst16[o] = (uint16_t)(jj[e] - bo);
-              else dst32[o] = jj[e];
-            }
+        for (int b = 0; b < CB; ++b) {
+          const int c = l + 4 * (m0 + b);
+          if (__ballot(c < nch) == 0) break;
+          int jj[4];
+          if constexpr (IDX16) {
+            jj[0] = bo + (int)(ch[b].x & 0xFFFFu); jj[1] = bo + (int)(ch[b].x >> 16);
+            jj[2] = bo + (int)(ch[b].y & 0xFFFFu); jj[3] = bo + (int)(ch[b].y >> 16);
+          } else {
+            jj[0] = (int)ch[b].x; jj[1] = (int)ch[b].y; jj[2] = (int)ch[b].z; jj[3] = (int)ch[b].w;
           }
-          cnt += __popc(bits);
+          bool ok[4];
+          float4 xj[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            ok[e] = 4 * c + e < m;
+            xj[e] = partner<LDS>(pin, s_pos, lds0, ok[e] ? jj[e] : self);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            // the reference's predicate without the minimum image, which is the
+            // identity here (a grid build whose extent reaches boxl/2 - margin
+            // raises ERR_PBC, and beads move less than the margin before the next)
+            const float dx = xj[e].x - xi.x, dy = xj[e].y - xi.y, dz = xj[e].z - xi.z;
+            const bool keep = ok[e] & (nlb | (((dx * dx + dy * dy) + dz * dz) < (w_type(xj[e].w) == 2 ? rcB : rcA)));
+            const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
+            if (keep) put(cnt + __popc(bits & below), jj[e]);
+            cnt += __popc(bits);
+          }
         }
       }
+      return cnt;
+    };
+    const int mrow = act ? nr : 0;
+    int cnt;
+    if constexpr (STAGE) {
+      cnt = pass(mrow, [&](int o, int j) {
+        if (o < FILTER_SROW) s_row[o] = (ent_t)(IDX16 ? j - bo : j);
+      });
+    } else {
+      cnt = pass(mrow, [](int, int) {});
     }
     // native pair list: natives within their pair cutoff, in native order
-    {
+    if (!nlb) {
       int ncnt = 0;
       for (int m0 = 0; __ballot(l + 4 * m0 < nn) != 0; m0 += NB) {
         if (m0 > 0) {
@@ -1674,53 +1692,62 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
       }
       if (act && l == 0) p.platt_cnt[i] = ncnt;
     }
-    if (act) {
-      const int c = min(cnt, pl_cap);
-      if (l == 0) {
-        p.plrep_cnt[i] = c;
-        if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
-      }
-      // pad to a multiple of ROW_PAD with the bead itself (pad_row)
-      for (int k = l; k < ((ROW_PAD - (c & (ROW_PAD - 1))) & (ROW_PAD - 1)); k += 4) {
-        if (c + k >= pl_cap) break;
-        if (STAGE && c + k < FILTER_SROW) s_row[c + k] = (uint16_t)(i - bo);
-        else if (IDX16) dst16[c + k] = (uint16_t)(i - bo);
-        else dst32[c + k] = i;
-      }
+    // row starts: the round's rows follow the rows packed so far, in slot order
+    const int c = act ? min(cnt, pl_cap) : 0;
+    const int len = (c + ROW_PAD - 1) & ~(ROW_PAD - 1);
+    const int incl = wave_incl_scan(l == 0 ? len : 0);
+    const int start = region + packed + incl - len;
+    packed += __shfl(incl, 63);
+    ent_t *dst = seg + start;
+    if (act && l == 0) {
+      p.plrep_cnt[i] = c;
+      p.plrep_off[i] = start;
+      if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
     }
-    if (STAGE) {
-      // the staged part of each row to global memory in 8-byte chunks (the
-      // row start is 32-byte aligned: pl_cap and pl_base are multiples of 16)
+    // pad to a multiple of ROW_PAD with the bead itself (its d2 = 0 fails the
+    // step kernel's repulsion guard)
+    const ent_t me = (ent_t)(IDX16 ? i - bo : i);
+    for (int k = c + l; k < len; k += 4) {
+      if (STAGE && k < FILTER_SROW) s_row[k] = me;
+      else dst[k] = me;
+    }
+    if constexpr (STAGE) {
+      // the staged part of each row to global memory in 8-byte chunks (row
+      // starts are multiples of ROW_PAD entries from a 64-byte aligned segment)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (act) {
-        const int c = min(cnt, pl_cap);
-        const int nst = min((c + ROW_PAD - 1) & ~(ROW_PAD - 1), FILTER_SROW) >> 2;
-        const uint2 *srow2 = reinterpret_cast<const uint2 *>(s_row);
-        uint2 *drow2 = reinterpret_cast<uint2 *>(dst16);
-        for (int k = l; k < nst; k += 4) drow2[k] = srow2[k];
-      }
+      constexpr int PER8 = 8 / (int)sizeof(ent_t);
+      const int nst = min(len, FILTER_SROW) / PER8;
+      const uint2 *srow2 = reinterpret_cast<const uint2 *>(s_row);
+      uint2 *drow2 = reinterpret_cast<uint2 *>(dst);
+      for (int k = l; k < nst; k += 4) drow2[k] = srow2[k];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // the entries not staged: a second pass over the rows that have them
+    const int lo = STAGE ? FILTER_SROW : 0;
+    if (__ballot(c > lo) != 0) {
+      pass(c > lo ? mrow : 0, [&](int o, int j) {
+        if (o >= lo && o < c) dst[o] = (ent_t)(IDX16 ? j - bo : j);
+      });
     }
   }
 }
 
 // Shell filter over its own work items (the step kernel's LDS staging, one
 // wave per tile): wave w of the block takes tile first + w of its item.
+// stage_off >= 0: rows are staged in LDS at this byte offset (FILTER_SROW
+// entries per bead of a round, per wave), for systems whose entries fit
+// stage_bytes (16-bit entries need FILTER_STAGE_BYTES, 32-bit twice that).
 template <bool KLDS>
-__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier, int stage_off) {
+__global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier, int stage_off,
+                                                               int stage_bytes) {
   extern __shared__ float4 s_pos[];
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // stage_off >= 0: rows of 16-bit entries are compacted in LDS at this byte
-  // offset (FILTER_SROW entries per bead of a round, per wave) and written out
-  // in 8-byte chunks; < 0: kept entries are stored one by one
-  uint16_t *s_rows = stage_off >= 0 ? reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(s_pos) + stage_off) +
-                                          wave * (64 / 4) * FILTER_SROW
-                                    : nullptr;
+  char *s_stage = reinterpret_cast<char *>(s_pos) + stage_off;
   int w0, w1;
   item_range<KLDS, true>(p, tier, w0, w1);
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
@@ -1736,11 +1763,18 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
     const int t = it.x + wave;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
-    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL) continue;
+    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL && S.build != BUILD_NL) continue;
     const float4 *pin = buf_in(p, S.parity + kstep);
-    if (!S.idx16) filter_tile<KLDS, false, false>(p, S, s, t, pin, s_pos, KLDS ? it.z : 0, nullptr);
-    else if (s_rows) filter_tile<KLDS, true, true>(p, S, s, t, pin, s_pos, KLDS ? it.z : 0, s_rows);
-    else filter_tile<KLDS, true, false>(p, S, s, t, pin, s_pos, KLDS ? it.z : 0, nullptr);
+    const int lds0 = KLDS ? it.z : 0;
+    const bool st16 = stage_off >= 0 && stage_bytes >= FILTER_STAGE_BYTES;
+    const bool st32 = stage_off >= 0 && stage_bytes >= 2 * FILTER_STAGE_BYTES;
+    if (S.idx16) {
+      if (st16) filter_tile<KLDS, true, true>(p, S, s, t, pin, s_pos, lds0, s_stage + wave * (FILTER_STAGE_BYTES / STEP_WAVES));
+      else filter_tile<KLDS, true, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
+    } else {
+      if (st32) filter_tile<KLDS, false, true>(p, S, s, t, pin, s_pos, lds0, s_stage + wave * (2 * FILTER_STAGE_BYTES / STEP_WAVES));
+      else filter_tile<KLDS, false, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
+    }
   }
 }
 
@@ -1844,7 +1878,7 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
             if (d / p.nat_r0[ns + k] < 1.25f) con += 1.0;
           }
         }
-        const int64_t pb = S.pl_base + (int64_t)(i - bead_off) * S.pl_cap;
+        const int64_t pb = S.pl_base + p.plrep_off[i];
         const int nr = p.plrep_cnt[i];
         for (int k = l; k < nr; k += LPB) {
           const int j = S.idx16 ? bead_off + ((const uint16_t *)(p.plrep_j + S.pl_base))[pb - S.pl_base + k]
@@ -2031,6 +2065,9 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
+  // neighbor-list builds: their rows before the shell filter packs them
+  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   const StepGrids &sg = lg.fg;
   const bool fork = fork_wanted(fk, sg);
   if (fork) {
@@ -2042,19 +2079,19 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   const bool stage0 = sg.lds + FILTER_STAGE_BYTES <= LDS_BYTES_MAX;
   if (sg.grid > 0)
     hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS),
-                       sg.lds + (stage0 ? FILTER_STAGE_BYTES : 0), st, p, kstep, 0, stage0 ? (int)sg.lds : -1);
+                       sg.lds + (stage0 ? FILTER_STAGE_BYTES : 0), st, p, kstep, 0, stage0 ? (int)sg.lds : -1,
+                       stage0 ? FILTER_STAGE_BYTES : 0);
   if (sg.grid_big > 0)
     hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, side, p, kstep, 1,
-                       -1);
+                       -1, 0);
+  // global-gather items stage no coordinates: room for 32-bit rows
   if (sg.grid_far > 0)
-    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), FILTER_STAGE_BYTES, side, p,
-                       kstep, 0, 0);
+    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 2 * FILTER_STAGE_BYTES, side,
+                       p, kstep, 0, 0, 2 * FILTER_STAGE_BYTES);
   if (fork) {
     const hipError_t h = fork_end(fk, st);
     if (h != hipSuccess) return h;
   }
-  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
-  hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   return SOPK_LAUNCH_CHECK();
 }
 
@@ -2113,7 +2150,7 @@ hipError_t step_lds_limit(size_t bytes) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES_MAX);
   if (h == hipSuccess)
     h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)FILTER_STAGE_BYTES);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * FILTER_STAGE_BYTES));
   return h;
 }
 

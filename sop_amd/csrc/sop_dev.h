@@ -112,10 +112,15 @@ struct DevPtrs {
   int32_t *nlatt_j; float *nlatt_r0; int32_t *nlatt_cnt; // native neighbor list (same slots)
   int2 *platt; int32_t *platt_cnt;  // native pair list (same slots): {partner, bits of r0}
   int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list
-  int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list: row i at pl_base +
-                                                         // (i - bead_off) * pl_cap, entries int32 arena
-                                                         // indices, or uint16 local indices (idx16)
-  int32_t *slrep_j; int32_t *slrep_cnt;                  // shell rows, same layout as the pair list
+  int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list, entries int32 arena indices
+                                                         // or uint16 local indices (idx16), rows packed per
+                                                         // tile (below)
+  int32_t *plrep_off;   // per bead: first entry of its pair-list row, counted from the system's segment
+                        // start (pl_base).  Tile k of a system owns entries [64 k pl_cap, 64 (k + 1) pl_cap)
+                        // of the segment; its rows lie one after another from the region's start in the
+                        // tile's bead order (tile_ord), each padded to a multiple of ROW_PAD entries.
+  int32_t *slrep_j; int32_t *slrep_cnt;                  // shell rows: row i at (i - bead_off) * pl_cap of the
+                                                         // segment, same entry format
   float4 *pos_shell;                                     // coordinates at the last shell-row build
   int32_t *cell_cnt, *cell_start, *cell_cursor;          // per cell
   int32_t *cell_of, *cell_tmp, *cell_sorted;             // per bead
