@@ -142,7 +142,7 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.cell_pos, 4},   {(void **)&p.nat_sorted, 8},
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
       {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16},
-      {(void **)&p.plrep_off, 4}};
+      {(void **)&p.plrep_off, 4},  {(void **)&p.platt_off, 4}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},    {(void **)&p.nat_r0, 4},  {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt, 8}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
@@ -393,7 +393,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   // with an upload stream, every upload goes through the pinned staging
   // buffer (a copy from pageable memory would wait for the engine stream)
   const size_t bl = (size_t)h.bead_len, nl_b = (size_t)h.nat_len;
-  const size_t stage_need = 32 * 256 + 16 * bl * 3 + 4 * bl * 8 + 8 * nl_b + bl + 4 * (bl / TILE) + sizeof(SysDev);
+  const size_t stage_need = 32 * 256 + 16 * bl * 3 + 4 * bl * 9 + 8 * nl_b + bl + 4 * (bl / TILE) + sizeof(SysDev);
   if (up && stage_need > upl->stage_bytes) return SOP_ERR_BUSY;
   sop_status st;
   // with an upload stream, free ranges only: growing a pool reallocates
@@ -496,7 +496,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   HIPCK(put(p.bond_r0 + b0, bond.data(), 4 * h.bead_len));
   HIPCK(put(p.nat_start + b0, nstart.data(), 4 * h.bead_len));
   HIPCK(put(p.nat_cnt + b0, ncnt.data(), 4 * h.bead_len));
-  for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt, p.plrep_off})
+  for (int32_t *arr : {p.nlatt_cnt, p.platt_cnt, p.nlrep_cnt, p.plrep_cnt, p.plrep_off, p.platt_off})
     HIPCK(put(arr + b0, zeros.data(), 4 * h.bead_len));
   // underdamped state: the velocities entering the first step (w = 0)
   if (d->vel)
@@ -1308,7 +1308,10 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
   int64_t seg_len = 0;
   if (which == SOP_LIST_NL_NATIVE || which == SOP_LIST_PL_NATIVE) {
     cnt_d = which == SOP_LIST_NL_NATIVE ? e->p.nlatt_cnt : e->p.platt_cnt;
-    HIPCK(hipMemcpy(start.data(), e->p.nat_start + h.bead_off, 4 * (size_t)nb, hipMemcpyDeviceToHost));
+    // the neighbor list's natives sit in each bead's native slots; the native
+    // pair list is packed per tile (platt_off)
+    HIPCK(hipMemcpy(start.data(), (which == SOP_LIST_NL_NATIVE ? e->p.nat_start : e->p.platt_off) + h.bead_off,
+                    4 * (size_t)nb, hipMemcpyDeviceToHost));
     ent.resize((size_t)h.nat_len);
     if (which == SOP_LIST_NL_NATIVE) {
       HIPCK(hipMemcpy(ent.data(), e->p.nlatt_j + h.nat_off, 4 * (size_t)h.nat_len, hipMemcpyDeviceToHost));

@@ -1121,7 +1121,10 @@ __global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
         }
         cnt += __popcll(m);
       }
-      if (lane == 0) p.platt_cnt[i] = cnt;
+      if (lane == 0) {
+        p.platt_cnt[i] = cnt;
+        p.platt_off[i] = ns;  // in the bead's own native slots
+      }
       const int64_t nb = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
       const int64_t pb = S.pl_base + (int64_t)(i - S.bead_off) * S.pl_cap;  // the bead's shell-row slot
       const int nr = p.nlrep_cnt[i];
@@ -1225,7 +1228,7 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   int t_ns = 0, t_cn = 0, t_po = 0;
   float t_br = 0.0f;  // r0 of bond (b0 + lane, b0 + lane + 1)
   if (b0 + lane < bend) {
-    t_ns = p.nat_start[b0 + lane];
+    t_ns = p.platt_off[b0 + lane];
     t_cn = p.platt_cnt[b0 + lane] | (p.plrep_cnt[b0 + lane] << 16);
     t_po = p.plrep_off[b0 + lane];
     t_br = p.bond_r0[b0 + lane];
@@ -1568,9 +1571,12 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   const double plf = S.pl_factor;
   const float pla = S.pl_abs;
   ent_t *const seg = reinterpret_cast<ent_t *>(p.plrep_j + pl_base);
-  // this tile's region of the segment, and the rows packed so far
+  // this tile's region of the segment, and the rows packed so far; the same
+  // for the native pair list, whose region is the tile's native CSR range
   const int region = (t * TILE - bo) * pl_cap;
   int packed = 0;
+  const int nat_region = __shfl(t_ns, 0);
+  int nat_packed = 0;
   for (int r = 0; r < 4; ++r) {
     const int li = __shfl(ordm, r * 16 + g);
     const int i = t * TILE + li;
@@ -1657,40 +1663,59 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     } else {
       cnt = pass(mrow, [](int, int) {});
     }
-    // native pair list: natives within their pair cutoff, in native order
+    // native pair list: natives within their pair cutoff, in native order,
+    // packed per tile like the rows (platt_off): the keep decisions first,
+    // then the entries at the round's starts.  The first batch's decisions
+    // are kept as bits; beads with more natives than one batch (rare) have
+    // every batch decided again when the entries are written.
     if (!nlb) {
-      int ncnt = 0;
-      for (int m0 = 0; __ballot(l + 4 * m0 < nn) != 0; m0 += NB) {
-        if (m0 > 0) {
-          // beads with more than 4 NB natives (rare): the next batch here
+      const bool multi = __ballot(nn > 4 * NB) != 0;
+      uint32_t km = 0;  // keep bits of the first batch
+      auto nat_pass = [&](bool wr, int o0) -> int {
+        int ncnt = 0;
+        for (int m0 = 0; __ballot(l + 4 * m0 < nn) != 0; m0 += NB) {
+          if (m0 > 0 || (wr && multi)) {
 #pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            const int k = l + 4 * (m0 + b);
-            nj[b] = self;
-            nr0[b] = 0.0f;
-            if (k < nn) {
-              nj[b] = p.nat_j[ns + k];
-              nr0[b] = p.nat_r0[ns + k];
+            for (int b = 0; b < NB; ++b) {
+              const int k = l + 4 * (m0 + b);
+              nj[b] = self;
+              nr0[b] = 0.0f;
+              if (k < nn) {
+                nj[b] = p.nat_j[ns + k];
+                nr0[b] = p.nat_r0[ns + k];
+              }
             }
           }
-        }
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const int k = l + 4 * (m0 + b);
-          const float4 xj = partner<LDS>(pin, s_pos, lds0, nj[b]);
-          const float rc = cutoff_of(plf, pla, nr0[b]);
-          // the reference's predicate without the minimum image (see above)
-          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-          const bool keep = (k < nn) & (((dx * dx + dy * dy) + dz * dz) < rc * rc);
-          const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
-          if (keep) {
-            const int o = ns + ncnt + __popc(bits & below);
-            p.platt[o] = make_int2(nj[b], __float_as_int(nr0[b]));
+          for (int b = 0; b < NB; ++b) {
+            bool keep;
+            if (wr && !multi) {
+              keep = (km >> b) & 1u;
+            } else {
+              const int k = l + 4 * (m0 + b);
+              const float4 xj = partner<LDS>(pin, s_pos, lds0, nj[b]);
+              const float rc = cutoff_of(plf, pla, nr0[b]);
+              // the reference's predicate without the minimum image (see above)
+              const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+              keep = (k < nn) & (((dx * dx + dy * dy) + dz * dz) < rc * rc);
+              if (!wr && m0 == 0 && keep) km |= 1u << b;
+            }
+            const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
+            if (wr && keep) p.platt[o0 + ncnt + __popc(bits & below)] = make_int2(nj[b], __float_as_int(nr0[b]));
+            ncnt += __popc(bits);
           }
-          ncnt += __popc(bits);
         }
+        return ncnt;
+      };
+      const int ncnt = nat_pass(false, 0);
+      const int nincl = wave_incl_scan(l == 0 ? ncnt : 0);
+      const int nstart = nat_region + nat_packed + nincl - ncnt;
+      nat_packed += __shfl(nincl, 63);
+      nat_pass(true, nstart);
+      if (act && l == 0) {
+        p.platt_cnt[i] = ncnt;
+        p.platt_off[i] = nstart;
       }
-      if (act && l == 0) p.platt_cnt[i] = ncnt;
     }
     // row starts: the round's rows follow the rows packed so far, in slot order
     const int c = act ? min(cnt, pl_cap) : 0;
@@ -1848,10 +1873,10 @@ __global__ __launch_bounds__(256) void k_energy(DevPtrs p) {
             }
           }
         }
-        const int ns = p.nat_start[i];
+        const int ns = p.nat_start[i], po = p.platt_off[i];
         const int nc = p.platt_cnt[i];
         for (int k = l; k < nc; k += LPB) {
-          const int2 pe = p.platt[ns + k];
+          const int2 pe = p.platt[po + k];
           const int j = pe.x;
           const float r0 = __int_as_float(pe.y);
           const float4 xj = pin[j];

@@ -110,7 +110,10 @@ struct DevPtrs {
   int32_t *nat_j;       // partner (arena index)
   float *nat_r0;
   int32_t *nlatt_j; float *nlatt_r0; int32_t *nlatt_cnt; // native neighbor list (same slots)
-  int2 *platt; int32_t *platt_cnt;  // native pair list (same slots): {partner, bits of r0}
+  int2 *platt; int32_t *platt_cnt;  // native pair list in the native slots: {partner, bits of r0}
+  int32_t *platt_off;   // per bead: first entry of its native pair list in platt.  A tile's lists lie
+                        // one after another from the tile's first native slot, in the tile's bead
+                        // order (neighbor-list builds: each in the bead's own native slots)
   int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list
   int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list, entries int32 arena indices
                                                          // or uint16 local indices (idx16), rows packed per
