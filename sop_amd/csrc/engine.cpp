@@ -200,7 +200,6 @@ static sop_status ensure_capacity(sop_engine *e, int64_t beads, int64_t nat, int
     const int64_t nn = next(c.nl, nl);
     sop_status s = grow_array(e, (void **)&e->p.nlrep_j, 4 * (size_t)c.nl, 4 * (size_t)nn);
     if (s) return s;
-    e->a_nl.grow(nn);
     c.nl = nn;
     changed = true;
   }
@@ -312,13 +311,15 @@ sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out) {
 
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
   const int64_t n = d->n_beads;
-  const int64_t nl_cap = round_up((int32_t)std::max<int64_t>(1, std::min<int64_t>(n - 1, e->opts.nl_cap_max)), 16);
   const int64_t pl_cap = round_up((int32_t)std::max<int64_t>(1, std::min<int64_t>(n - 1, e->opts.pl_cap_max)), 16);
   const int64_t beads = (n + TILE - 1) / TILE * TILE;
   const int64_t cell_cap = std::min<int64_t>(e->opts.cell_cap_max, std::max<int64_t>(64, 2 * n)) + 1;
   int64_t per_bead = 4;  // tile table share
   for (const auto &a : e->bead_arrays) per_bead += (int64_t)a.second;
-  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 4 * n * (nl_cap + 2 * pl_cap) + 12 * cell_cap;
+  // pair-list and shell-row segments (16-bit entries up to 65,536 beads); the
+  // neighbor list is materialized only on request, in a scratch segment
+  const int64_t esz = n <= 65536 ? 2 : 4;
+  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 2 * esz * n * pl_cap + 12 * cell_cap;
 }
 
 // Validate and upload one system, without its first lists (engine_init_lists).
@@ -381,8 +382,8 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   h.n_nat = nn;
   h.bead_len = round_up(n, TILE);
   h.nat_len = std::max(1, 2 * nn);
-  h.nl_len = (int64_t)n * nl_cap;
-  h.pl_len = (int64_t)n * pl_cap;
+  h.nl_len = (int64_t)n * nl_cap;  // entries when materialized (scratch)
+  h.pl_len = (int64_t)n * pl_cap / (n <= 65536 ? 2 : 1);  // 32-bit words: 16-bit entries pack two a word
   h.cell_len = cell_cap + 1;
   h.nl_cap = nl_cap;
   h.pl_cap = pl_cap;
@@ -412,18 +413,13 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
     return set_error(e, SOP_ERR_CAPACITY, "native pool beyond 4 GiB");
   }
-  h.nl_off = alloc(e->a_nl, h.nl_len, 16, 2);
-  if (st) { e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len); return st; }
+  h.nl_off = 0;
   h.pl_off = alloc(e->a_pl, h.pl_len, 16, 3);
-  if (st) {
-    e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
-    e->a_nl.release(h.nl_off, h.nl_len);
-    return st;
-  }
+  if (st) { e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len); return st; }
   h.cell_off = alloc(e->a_cells, h.cell_len, 1, 4);
   if (st) {
     e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
-    e->a_nl.release(h.nl_off, h.nl_len); e->a_pl.release(h.pl_off, h.pl_len);
+    e->a_pl.release(h.pl_off, h.pl_len);
     return st;
   }
   // slot
@@ -432,7 +428,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     if (!e->sys[s].used) { slot = (int32_t)s; break; }
   if (slot < 0 && up) {
     e->a_beads.release(h.bead_off, h.bead_len); e->a_nat.release(h.nat_off, h.nat_len);
-    e->a_nl.release(h.nl_off, h.nl_len); e->a_pl.release(h.pl_off, h.pl_len);
+    e->a_pl.release(h.pl_off, h.pl_len);
     e->a_cells.release(h.cell_off, h.cell_len);
     return SOP_ERR_BUSY;
   }
@@ -515,7 +511,7 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   S.nl_cap = nl_cap;
   S.pl_cap = pl_cap;
   S.idx16 = n <= 65536 ? 1 : 0;
-  S.nl_base = h.nl_off;
+  S.nl_base = 0;  // the neighbor-list scratch, when materialized
   S.pl_base = h.pl_off;
   S.cell_base = (int32_t)h.cell_off;
   S.cell_cap = cell_cap;
@@ -642,8 +638,8 @@ sop_status engine_remove_system(sop_engine *e, int32_t slot) {
   SysHost &h = e->sys[slot];
   e->a_beads.release(h.bead_off, h.bead_len);
   e->a_nat.release(h.nat_off, h.nat_len);
-  e->a_nl.release(h.nl_off, h.nl_len);
   e->a_pl.release(h.pl_off, h.pl_len);
+  if (e->nl_owner == slot) e->nl_owner = -1;
   e->a_cells.release(h.cell_off, h.cell_len);
   h = SysHost();
   return SOP_OK;
@@ -1189,15 +1185,15 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     return SOP_ERR_HIP;
   }
   if (e->opts.pool_bytes > 0) {
-    // fixed pools sized from the budget (serving): split by typical use
-    // (per bead ~152 B + 4*(nl_cap+pl_cap) B; lists dominate)
+    // fixed pools sized from the budget (serving): split by typical use (per
+    // bead ~200 B + 2 * pl_cap entries of 2 B; the pair-list and shell-row
+    // segments dominate); the neighbor-list scratch grows on request
     const int64_t B = e->opts.pool_bytes;
-    const int64_t beads = std::max<int64_t>(4096, B / 8 / 152) / TILE * TILE;
+    const int64_t beads = std::max<int64_t>(4096, B / 8 / 200) / TILE * TILE;
     const int64_t nat = std::max<int64_t>(4096, B / 64 / 24);
-    const int64_t nl = std::max<int64_t>(65536, B * 5 / 8 / 4);
-    const int64_t pl = std::max<int64_t>(65536, B / 8 / 4);
+    const int64_t pl = std::max<int64_t>(65536, B * 3 / 4 / 8);
     const int64_t cells = std::max<int64_t>(65536, B / 64 / 12);
-    sop_status st = ensure_capacity(e, beads, nat, nl, pl, cells, 4096);
+    sop_status st = ensure_capacity(e, beads, nat, 0, pl, cells, 4096);
     if (st) {
       sop_engine_destroy(e);
       return st;
@@ -1265,9 +1261,24 @@ sop_status sop_system_info(sop_engine *e, int32_t sys, int32_t *n_beads, int64_t
   return SOP_OK;
 }
 
+// The neighbor list is written only on request, into one scratch segment
+// (grown to fit) that holds the rows of the last system materialized.
+static sop_status nl_scratch_for(sop_engine *e, int32_t sys) {
+  sop_status st = ensure_capacity(e, 0, 0, e->sys[sys].nl_len, 0, 0, 0);
+  if (st) return st;
+  if (e->nl_owner != sys) {  // the scratch holds another system's rows
+    const int32_t zero = 0;
+    HIPCK(hipMemcpy(&e->p.sys[sys].nl_valid, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  e->nl_owner = sys;
+  return SOP_OK;
+}
+
 sop_status sop_update_neighbor_list(sop_engine *e, int32_t sys) {
   if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
-  sop_status st = engine_set_batch(e, &sys, 1);
+  sop_status st = nl_scratch_for(e, sys);
+  if (st) return st;
+  st = engine_set_batch(e, &sys, 1);
   if (st) return st;
   HIPCK(launch_lists(e->p, list_grids(e), 0, FORCE_NL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
@@ -1285,7 +1296,9 @@ sop_status sop_update_pair_list(sop_engine *e, int32_t sys) {
 
 // The neighbor list is materialized on demand from its snapshot.
 static sop_status materialize_nl(sop_engine *e, int32_t sys) {
-  sop_status st = engine_set_batch(e, &sys, 1);
+  sop_status st = nl_scratch_for(e, sys);
+  if (st) return st;
+  st = engine_set_batch(e, &sys, 1);
   if (st) return st;
   HIPCK(launch_lists(e->p, list_grids(e), 0, MATERIALIZE_NL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
@@ -1324,7 +1337,7 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
   } else if (which == SOP_LIST_NL_REP || which == SOP_LIST_PL_REP) {
     const bool nl = which == SOP_LIST_NL_REP;
     cnt_d = nl ? e->p.nlrep_cnt : e->p.plrep_cnt;
-    seg_len = nl ? h.nl_len : h.pl_len;
+    seg_len = nl ? h.nl_len : (int64_t)h.n * h.pl_cap;  // entries
     ent.resize((size_t)seg_len);
     if (!nl && h.n <= 65536) {
       // 16-bit local indices from the segment start

@@ -81,7 +81,8 @@ struct sop_engine {
   bool fixed_pools = false;
 
   sopk::Pools cap;
-  sopk::RangeAlloc a_beads, a_nat, a_nl, a_pl, a_cells;
+  sopk::RangeAlloc a_beads, a_nat, a_pl, a_cells;
+  int32_t nl_owner = -1;  // system whose neighbor list the scratch holds
   std::vector<sopk::SysHost> sys;  // by slot
   sopk::DevPtrs p{};
   // raw buffers in allocation order (for growth/free)

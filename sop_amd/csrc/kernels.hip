@@ -1,14 +1,15 @@
 // kernels.hip -- gfx950 kernels of the SOP overdamped hot path.
 //
 //   k_bbox / k_cell_count / k_cell_scan / k_cell_scatter / k_cell_rank /
-//   k_nl_build   neighbor-list rebuild: per-system open cell grid, counting
-//                sort of beads into cells (histogram, DPP wave scan, scatter,
-//                deterministic in-cell ranking), 27-cell stencil with the exact
-//                FP32 predicate and ballot/mbcnt wave compaction.  Replaces the
-//                reference's flag kernels + CUDPP split/scan over the O(N^2)
-//                master list (sop.cu:704-826, 928-1262, 1827-2608).
-//   k_pl_build   pair-list filter of the neighbor list, same compaction
-//                (sop.cu:835-920, 1456-1620).
+//   k_pl_cells / k_nl_build / k_shell_filter   list rebuilds: per-system open
+//                cell grid, counting sort of beads into cells (histogram, DPP
+//                wave scan, scatter, deterministic in-cell ranking), 27-cell
+//                stencil with the exact FP32 predicates and ballot/mbcnt wave
+//                compaction, pair-list rows filtered from shell rows and packed
+//                per tile.  Replaces the reference's flag kernels + CUDPP
+//                split/scan over the O(N^2) master list and its pair-list
+//                filter (sop.cu:704-826, 835-920, 928-1262, 1456-1620,
+//                1827-2608).
 //   k_step       fused FENE + i,i+2 soft sphere + native LJ + non-native
 //                repulsion gather (no atomics), group reduction over the lanes
 //                of a bead, Philox4x32-10 random force, overdamped move
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(SKIN_THREADS) void k_skin(DevPtrs p, int kstep, int
       }
       S.build = build;
       if (build == BUILD_DIRECT) S.sl_valid = 1;
-      if (build == BUILD_NL) S.sl_valid = 0;  // k_pl_build writes its rows into the shell-row slots
+      if (build == BUILD_NL) S.sl_valid = 0;  // k_nl_build writes its rows into the shell-row slots
       if (build == BUILD_DIRECT || build == BUILD_SHELL || build == BUILD_NL)
         ++S.nbuild[build == BUILD_DIRECT ? 0 : (build == BUILD_SHELL ? 1 : 2)];
       s_build = build;
@@ -527,9 +528,10 @@ __device__ __forceinline__ void natbits_mark(uint32_t *bits, const int32_t *natj
 // with d2 below the cutoff of their type pair (rcA: tj = 1, rcB: tj = 2) are
 // appended to out[0..cap) with ballot/mbcnt
 // compaction; returns the count (which may exceed cap).
+template <typename Extra, typename Emit>
 __device__ __forceinline__ int scan_cells(const DevPtrs &p, const SysDev &S, int i, float4 xi,
-                                          float rcA, float rcB, int ns, int nn, int32_t *out, int cap,
-                                          uint32_t *bits) {
+                                          float rcA, float rcB, int ns, int nn, uint32_t *bits, Extra extra,
+                                          Emit emit) {
   const int lane = threadIdx.x & 63;
   const int wi = __float_as_int(xi.w), ci = wi >> 2;
   const float boxl = S.boxl;
@@ -561,14 +563,11 @@ __device__ __forceinline__ int scan_cells(const DevPtrs &p, const SysDev &S, int
             const int jl = (j - S.bead_off) & 0xFFFF;
             bool nat = (bits[jl >> 5] >> (jl & 31)) & 1u;
             if (nat && !exact_key) nat = native_pair(natj, nn, j);
-            pred = !nat;
+            pred = !nat && extra(j, wj);
           }
         }
         const uint64_t mk = __ballot(pred);
-        if (pred) {
-          const int o = cnt + lanes_below(mk);
-          if (o < cap) out[o] = j;
-        }
+        if (pred) emit(cnt + lanes_below(mk), j);
         cnt += __popcll(mk);
       }
     }
@@ -1023,8 +1022,15 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
   }
 }
 
-// Wave per bead: materialize the neighbor list of the snapshot pos_ref.
-__global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
+// Wave per bead, from the cell grid of the neighbor-list snapshot pos_ref.
+// BUILD_NL_ONLY: materialize the neighbor list (pairs within rc_nl at
+// pos_ref) into the neighbor-list scratch.  BUILD_NL: the pair list of the
+// reference's two-level scheme, neighbor-list pairs within rc_pl at x, in one
+// pass without materializing the neighbor list: the native pair list in each
+// bead's native slots, the non-native rows in the shell-row slots (the shell
+// filter packs them into the pair list), which therefore stop holding shell
+// rows (k_skin clears sl_valid).
+__global__ __launch_bounds__(256) void k_nl_build(DevPtrs p, int kstep) {
   if (*p.nlbuild == 0) return;  // no system of the batch needs it (k_cellwork)
   __shared__ uint32_t s_bits[4][NATBITS_WORDS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1037,12 +1043,16 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
     const int s = p.tile_sys[t];
     const SysDev &S = p.sys[s];
     if (S.build != BUILD_NL && S.build != BUILD_NL_ONLY) continue;
+    const bool plb = S.build == BUILD_NL;
     const float4 *ref = p.pos_ref;
+    const float4 *pin = buf_in(p, S.parity + kstep);
+    const float boxl = S.boxl;
     const int bend = min(t * TILE + TILE, S.bead_off + S.n);
     for (int q = 0; q < TILE / 4; ++q) {
       const int i = t * TILE + wave * (TILE / 4) + q;
       if (i >= bend) break;
       const float4 xi = ref[i];
+      const float4 xn = pin[i];
       const int ti = __float_as_int(xi.w) & 3;
       const int ns = p.nat_start[i], nn = p.nat_cnt[i];
       int cnt = 0;
@@ -1055,103 +1065,62 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p) {
           j = p.nat_j[ns + k];
           r0 = p.nat_r0[ns + k];
           const float rc = cutoff_of(S.nl_factor, S.nl_abs, r0);
-          pred = dist2_exact(xi, ref[j], S.boxl) < rc * rc;
-        }
-        const uint64_t m = __ballot(pred);
-        if (pred) {
-          const int o = ns + cnt + lanes_below(m);
-          p.nlatt_j[o] = j;
-          p.nlatt_r0[o] = r0;
-        }
-        cnt += __popcll(m);
-      }
-      if (lane == 0) p.nlatt_cnt[i] = cnt;
-      const int64_t base = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
-      const float rcA = S.rc2_nl_rep[(ti - 1) * 2], rcB = S.rc2_nl_rep[(ti - 1) * 2 + 1];
-      const int cr = scan_cells(p, S, i, xi, rcA, rcB, ns, nn, p.nlrep_j + base, S.nl_cap, s_bits[wave]);
-      if (lane == 0) {
-        p.nlrep_cnt[i] = min(cr, S.nl_cap);
-        if (cr > S.nl_cap) atomicOr(&p.sys[s].err, ERR_NL_OVF);
-        p.sys[s].nl_valid = 1;
-      }
-    }
-  }
-}
-
-// Wave per bead: pair list = materialized neighbor list filtered at x.  The
-// native pair list is written in place; the non-native rows go to the
-// shell-row slots (the shell filter packs them into the pair list), which
-// therefore stop holding shell rows (k_skin clears sl_valid).
-__global__ __launch_bounds__(256) void k_pl_build(DevPtrs p, int kstep) {
-  if (*p.nlbuild == 0) return;  // no system of the batch needs it (k_cellwork)
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nwork = *p.nwork;
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const int t = p.work[w];
-    if (t < 0) continue;
-    const int s = p.tile_sys[t];
-    const SysDev &S = p.sys[s];
-    if (S.build != BUILD_NL) continue;
-    const float4 *pin = buf_in(p, S.parity + kstep);
-    const float boxl = S.boxl;
-    const int bend = min(t * TILE + TILE, S.bead_off + S.n);
-    for (int q = 0; q < TILE / 4; ++q) {
-      const int i = t * TILE + wave * (TILE / 4) + q;
-      if (i >= bend) break;
-      const float4 xi = pin[i];
-      const int ti = __float_as_int(xi.w) & 3;
-      const int ns = p.nat_start[i];
-      const int na = p.nlatt_cnt[i];
-      int cnt = 0;
-      for (int k0 = 0; k0 < na; k0 += 64) {
-        const int k = k0 + lane;
-        bool pred = false;
-        int j = 0;
-        float r0 = 0.0f;
-        if (k < na) {
-          j = p.nlatt_j[ns + k];
-          r0 = p.nlatt_r0[ns + k];
-          const float rc = cutoff_of(S.pl_factor, S.pl_abs, r0);
-          pred = dist2_exact(xi, pin[j], boxl) < rc * rc;
-        }
-        const uint64_t m = __ballot(pred);
-        if (pred) {
-          const int o = ns + cnt + lanes_below(m);
-          p.platt[o] = make_int2(j, __float_as_int(r0));
-        }
-        cnt += __popcll(m);
-      }
-      if (lane == 0) {
-        p.platt_cnt[i] = cnt;
-        p.platt_off[i] = ns;  // in the bead's own native slots
-      }
-      const int64_t nb = S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
-      const int64_t pb = S.pl_base + (int64_t)(i - S.bead_off) * S.pl_cap;  // the bead's shell-row slot
-      const int nr = p.nlrep_cnt[i];
-      const float rcA = S.rc2_pl_rep[(ti - 1) * 2], rcB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
-      int cr = 0;
-      for (int m0 = 0; m0 < nr; m0 += 64) {
-        const int m = m0 + lane;
-        bool pred = false;
-        int j = 0;
-        if (m < nr) {
-          j = p.nlrep_j[nb + m];
-          const float4 xj = pin[j];
-          pred = dist2_exact(xi, xj, boxl) < (w_type(xj.w) == 2 ? rcB : rcA);
-        }
-        const uint64_t mk = __ballot(pred);
-        if (pred) {
-          const int o = cr + lanes_below(mk);
-          if (o < S.pl_cap) {
-            if (S.idx16) ((uint16_t *)(p.slrep_j + S.pl_base))[pb - S.pl_base + o] = (uint16_t)(j - S.bead_off);
-            else p.slrep_j[pb + o] = j;
+          pred = dist2_exact(xi, ref[j], boxl) < rc * rc;
+          if (plb && pred) {
+            const float rp = cutoff_of(S.pl_factor, S.pl_abs, r0);
+            pred = dist2_exact(xn, pin[j], boxl) < rp * rp;
           }
         }
-        cr += __popcll(mk);
+        const uint64_t m = __ballot(pred);
+        if (pred) {
+          const int o = ns + cnt + lanes_below(m);
+          if (plb) {
+            p.platt[o] = make_int2(j, __float_as_int(r0));
+          } else {
+            p.nlatt_j[o] = j;
+            p.nlatt_r0[o] = r0;
+          }
+        }
+        cnt += __popcll(m);
       }
-      if (lane == 0) {
-        p.slrep_cnt[i] = min(cr, S.pl_cap);
-        if (cr > S.pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+      const float rcA = S.rc2_nl_rep[(ti - 1) * 2], rcB = S.rc2_nl_rep[(ti - 1) * 2 + 1];
+      if (plb) {
+        if (lane == 0) {
+          p.platt_cnt[i] = cnt;
+          p.platt_off[i] = ns;  // in the bead's own native slots
+        }
+        const float pA = S.rc2_pl_rep[(ti - 1) * 2], pB = S.rc2_pl_rep[(ti - 1) * 2 + 1];
+        const int64_t row = (int64_t)(i - S.bead_off) * S.pl_cap;  // the bead's shell-row slot
+        uint16_t *r16 = (uint16_t *)(p.slrep_j + S.pl_base) + row;
+        int32_t *r32 = p.slrep_j + S.pl_base + row;
+        const int cap = S.pl_cap, bo = S.bead_off;
+        const bool i16 = S.idx16 != 0;
+        const int cr = scan_cells(
+            p, S, i, xi, rcA, rcB, ns, nn, s_bits[wave],
+            [&](int j, int wj) { return dist2_exact(xn, pin[j], boxl) < ((wj & 3) == 2 ? pB : pA); },
+            [&](int o, int j) {
+              if (o < cap) {
+                if (i16) r16[o] = (uint16_t)(j - bo);
+                else r32[o] = j;
+              }
+            });
+        if (lane == 0) {
+          p.slrep_cnt[i] = min(cr, cap);
+          if (cr > cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+        }
+      } else {
+        if (lane == 0) p.nlatt_cnt[i] = cnt;
+        int32_t *out = p.nlrep_j + S.nl_base + (int64_t)(i - S.bead_off) * S.nl_cap;
+        const int cap = S.nl_cap;
+        const int cr = scan_cells(p, S, i, xi, rcA, rcB, ns, nn, s_bits[wave], [](int, int) { return true; },
+                                  [&](int o, int j) {
+                                    if (o < cap) out[o] = j;
+                                  });
+        if (lane == 0) {
+          p.nlrep_cnt[i] = min(cr, cap);
+          if (cr > cap) atomicOr(&p.sys[s].err, ERR_NL_OVF);
+          p.sys[s].nl_valid = 1;
+        }
       }
     }
   }
@@ -1518,8 +1487,8 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
 // Pair list of one tile's beads = their shell rows filtered at x with the
 // reference's exact predicate (d2 < rc_pl^2, squares summed in order without
 // contraction), for grid and shell builds, and the native pair list from the
-// native CSR.  Neighbor-list builds: k_pl_build has left the filtered rows in
-// the shell-row slots and written the native pair list; they are packed
+// native CSR.  Neighbor-list builds: k_nl_build has left the filtered rows in
+// the shell-row slots and written the native pair list; the rows are packed
 // here as they are.  4 lanes per bead, 16 beads per round: lane l
 // takes 4-entry chunks l, l + 4, ... of the row, kept entries are compacted
 // per chunk position, so a row comes out in (chunk pass, position, lane)
@@ -1542,7 +1511,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   const int bo = S.bead_off, pl_cap = S.pl_cap;
   const int64_t pl_base = S.pl_base;
   const int bend = min(t * TILE + TILE, bo + S.n);
-  const bool nlb = S.build == BUILD_NL;  // neighbor-list build: rows already filtered (k_pl_build)
+  const bool nlb = S.build == BUILD_NL;  // neighbor-list build: rows already filtered (k_nl_build)
   // cutoffs as wave-uniform values (a per-lane indexed load would be a vector
   // load, and waiting for it would also wait for the earlier stores)
   const float rc11 = S.rc2_pl_rep[0], rc12 = S.rc2_pl_rep[1], rc21 = S.rc2_pl_rep[2], rc22 = S.rc2_pl_rep[3];
@@ -2091,8 +2060,7 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
   // neighbor-list builds: their rows before the shell filter packs them
-  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p);
-  hipLaunchKernelGGL(k_pl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
+  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
   const StepGrids &sg = lg.fg;
   const bool fork = fork_wanted(fk, sg);
   if (fork) {

@@ -32,7 +32,7 @@ struct SysDev {
   int32_t nl_cap;    // per-bead capacity of the non-native neighbor list
   int32_t pl_cap;    // per-bead capacity of the non-native pair list
   int32_t idx16;     // 1: pair-list rows hold 16-bit local indices (n <= 65536)
-  int64_t nl_base;   // first entry of this system's neighbor-list segment
+  int64_t nl_base;   // first entry of this system's neighbor-list rows in the scratch (materialized on request)
   int64_t pl_base;   // first entry of this system's pair-list segment
   int32_t cell_base; // first cell of this system's cell segment (cell_cap+1 slots)
   int32_t cell_cap;
@@ -91,7 +91,7 @@ struct SysDev {
 // SysDev::build
 constexpr int BUILD_NONE = 0;    // no list work at this step
 constexpr int BUILD_DIRECT = 1;  // pair list straight from a pair-cutoff cell grid
-constexpr int BUILD_NL = 2;      // materialize the neighbor list from pos_ref, then filter
+constexpr int BUILD_NL = 2;      // pair list = neighbor list of pos_ref filtered at x, in one pass
 constexpr int BUILD_NL_ONLY = 3; // materialize the neighbor list only
 constexpr int BUILD_SHELL = 4;   // pair list = shell rows filtered at x (no cell grid)
 
@@ -114,7 +114,8 @@ struct DevPtrs {
   int32_t *platt_off;   // per bead: first entry of its native pair list in platt.  A tile's lists lie
                         // one after another from the tile's first native slot, in the tile's bead
                         // order (neighbor-list builds: each in the bead's own native slots)
-  int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list
+  int32_t *nlrep_j; int32_t *nlrep_cnt;                  // non-native neighbor list (nlrep_j: scratch of the
+                                                         // last system materialized)
   int32_t *plrep_j; int32_t *plrep_cnt;                  // non-native pair list, entries int32 arena indices
                                                          // or uint16 local indices (idx16), rows packed per
                                                          // tile (below)
