@@ -233,11 +233,22 @@ typedef struct sop_list_stats {
   int64_t nl_native, nl_rep, pl_native, pl_rep; /* half-list entry counts */
   int32_t nl_max_per_bead, pl_max_per_bead;
   /* pair-list rebuilds since the system was added, by how they were made: from
-   * the cell grid (with new shell rows), by filtering the shell rows, or by
-   * filtering the materialized neighbor list */
+   * the cell grid (with new shell rows), by filtering the shell rows, or from
+   * the neighbor-list snapshot's cell grid (beads had left the skin) */
   int64_t grid_builds, shell_builds, nl_builds;
 } sop_list_stats;
 sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *st);
+/* Row capacity of a system's shell rows and pair-list tiles (entries per
+ * bead) and the longest row built so far.  The capacity starts at
+ * min(n - 1, pl_cap_max, 384), is fitted to the first lists, and grows after
+ * runs whose rows came within an eighth of it; growth moves the rows and
+ * changes no result.  No counterpart in the reference (whose lists are sized
+ * by the O(N^2) master list). */
+sop_status sop_system_rows(sop_engine *e, int32_t sys, int32_t *capacity, int32_t *longest);
+/* Set that capacity ahead of a change the engine cannot foresee (a system
+ * about to compact): rounded up to a multiple of 16, kept within
+ * [longest row, min(n - 1, pl_cap_max)].  Results do not change. */
+sop_status sop_system_reserve_rows(sop_engine *e, int32_t sys, int32_t capacity);
 /* Time `reps` launches of the fused force+integrate kernel over the listed
  * systems with HIP events on the engine stream; avg_ms per launch.  The
  * systems' coordinates and counters are restored afterwards. */

@@ -329,6 +329,18 @@ class Engine:
         self._c(lib().sop_list_stats_get(self._h, sys, C.byref(s)))
         return s
 
+    def rows(self, sys):
+        """(row capacity, longest row so far) of a system's shell rows and
+        pair-list tiles, in entries per bead."""
+        cap, longest = C.c_int32(), C.c_int32()
+        self._c(lib().sop_system_rows(self._h, sys, C.byref(cap), C.byref(longest)))
+        return cap.value, longest.value
+
+    def reserve_rows(self, sys, capacity: int):
+        """Set a system's row capacity (kept within [longest row, min(n - 1,
+        pl_cap_max)]); results do not change."""
+        self._c(lib().sop_system_reserve_rows(self._h, sys, capacity))
+
     def time_step_kernel(self, systems, reps=20) -> float:
         a = np.ascontiguousarray(np.atleast_1d(systems), np.int32)
         ms = C.c_double()

@@ -144,6 +144,8 @@ SIGNATURES = {
     "sop_set_rng": (i32, [vp, i32, C.c_uint64, C.c_int64]),
     "sop_get_increments": (i32, [vp, i32, P(f32)]),
     "sop_list_stats_get": (i32, [vp, i32, P(ListStats)]),
+    "sop_system_rows": (i32, [vp, i32, P(C.c_int32), P(C.c_int32)]),
+    "sop_system_reserve_rows": (i32, [vp, i32, i32]),
     "sop_time_step_kernel": (i32, [vp, P(i32), i32, i32, P(f64)]),
     "sop_autotune": (i32, [vp, P(i32), i32, P(i32), P(i32)]),
     "sop_autotune_ex": (i32, [vp, P(i32), i32, P(TuneResult)]),

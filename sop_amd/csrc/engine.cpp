@@ -146,7 +146,7 @@ static void bind_arrays(sop_engine *e) {
   e->nat_arrays = {{(void **)&p.nat_j, 4},    {(void **)&p.nat_r0, 4},  {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt, 8}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
-                    {(void **)&p.cellitems, 4}};
+                    {(void **)&p.cellitems, 8}};
 }
 
 static sop_status grow_array(sop_engine *e, void **ptr, size_t old_bytes, size_t new_bytes) {
@@ -269,6 +269,23 @@ static float cutoff_host(double factor, float absval, float base) {
 
 static inline int32_t round_up(int32_t v, int32_t a) { return (v + a - 1) / a * a; }
 
+// Row capacity of a system's shell rows and pair-list tiles (entries per
+// bead): it starts at PL_CAP_START, is fitted to the longest row of the first
+// build, and grows when a rebuild's longest row comes within an eighth of it,
+// up to min(n - 1, pl_cap_max).  Growth moves the rows and changes no result.
+constexpr int32_t PL_CAP_START = 384;
+
+static int32_t pl_cap_limit(const sop_engine *e, int32_t n) {
+  return round_up(std::max(1, std::min(n - 1, e->opts.pl_cap_max)), 16);
+}
+static int32_t pl_cap_start(const sop_engine *e, int32_t n) { return std::min(pl_cap_limit(e, n), PL_CAP_START); }
+static int32_t pl_cap_target(const sop_engine *e, int32_t n, int32_t longest) {
+  return std::min(pl_cap_limit(e, n), round_up(longest + longest / 4 + 32, 16));
+}
+static bool pl_cap_short(const sop_engine *e, const SysHost &h, int32_t longest) {
+  return h.pl_cap < pl_cap_limit(e, h.n) && longest > h.pl_cap - h.pl_cap / 8;
+}
+
 namespace sopk {
 
 // Pair-list shell of a system: rows within rc_pl + shell of pos_shell, valid
@@ -311,7 +328,7 @@ sop_status engine_read_sys(sop_engine *e, int32_t slot, SysDev *out) {
 
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
   const int64_t n = d->n_beads;
-  const int64_t pl_cap = round_up((int32_t)std::max<int64_t>(1, std::min<int64_t>(n - 1, e->opts.pl_cap_max)), 16);
+  const int64_t pl_cap = pl_cap_start(e, d->n_beads);
   const int64_t beads = (n + TILE - 1) / TILE * TILE;
   const int64_t cell_cap = std::min<int64_t>(e->opts.cell_cap_max, std::max<int64_t>(64, 2 * n)) + 1;
   int64_t per_bead = 4;  // tile table share
@@ -319,7 +336,9 @@ int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d) {
   // pair-list and shell-row segments (16-bit entries up to 65,536 beads); the
   // neighbor list is materialized only on request, in a scratch segment
   const int64_t esz = n <= 65536 ? 2 : 4;
-  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 2 * esz * n * pl_cap + 12 * cell_cap;
+  int64_t per_cell = 0;
+  for (const auto &a : e->cell_arrays) per_cell += (int64_t)a.second;
+  return beads * per_bead + 24 * 2 * (int64_t)d->n_native + 2 * esz * n * pl_cap + per_cell * cell_cap;
 }
 
 // Validate and upload one system, without its first lists (engine_init_lists).
@@ -369,9 +388,10 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   }
   // capacities
   const int32_t nl_cap = round_up(std::max(1, std::min(n - 1, e->opts.nl_cap_max)), 16);
-  const int32_t pl_cap = round_up(std::max(1, std::min(n - 1, e->opts.pl_cap_max)), 16);
-  // the step kernel addresses a system's pair-list rows with 32-bit byte offsets
-  if ((int64_t)n * pl_cap * (n <= 65536 ? 2 : 4) > (int64_t)0xFFFFFFFF)
+  const int32_t pl_cap = pl_cap_start(e, n);
+  // the step kernel addresses a system's pair-list rows with 32-bit byte
+  // offsets, at any capacity they may grow to
+  if ((int64_t)n * pl_cap_limit(e, n) * (n <= 65536 ? 2 : 4) > (int64_t)0xFFFFFFFF)
     return set_error(e, SOP_ERR_CAPACITY, "system's pair-list rows exceed 4 GiB (lower pl_cap_max)");
   const int32_t cell_cap = std::min(e->opts.cell_cap_max, std::max(64, 2 * n));
   int32_t dmax = 1;
@@ -604,15 +624,142 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
   return SOP_OK;
 }
 
+static sop_status read_sys_all(sop_engine *e, std::vector<SysDev> &sd) {
+  sd.resize((size_t)e->cap.slots);
+  HIPCK(hipMemcpy(sd.data(), e->p.sys, sizeof(SysDev) * sd.size(), hipMemcpyDeviceToHost));
+  return SOP_OK;
+}
+
+// Resident bytes of one system (what job_bytes_estimate predicts for it).
+int64_t engine_system_bytes(const sop_engine *e, int32_t slot) {
+  const SysHost &h = e->sys[slot];
+  int64_t per_bead = 4;  // tile table share
+  for (const auto &a : e->bead_arrays) per_bead += (int64_t)a.second;
+  int64_t per_cell = 0;
+  for (const auto &a : e->cell_arrays) per_cell += (int64_t)a.second;
+  return h.bead_len * per_bead + 24 * 2 * (int64_t)h.n_nat + 2 * 4 * h.pl_len + per_cell * h.cell_len;
+}
+
+// Move one system's shell rows and pair-list tiles to a segment of row
+// capacity cap: with keep, the current rows are copied over (k_relayout);
+// otherwise the caller builds them again.  Leaves the engine stream idle.
+sop_status engine_resize_rows(sop_engine *e, int32_t slot, int32_t cap, bool keep) {
+  SysHost &h = e->sys[slot];
+  if (cap == h.pl_cap) return SOP_OK;
+  const bool i16 = h.n <= 65536;
+  const int64_t len = (int64_t)h.n * cap / (i16 ? 2 : 1);
+  sop_status st = SOP_OK;
+  const int64_t off = alloc_or_grow(e, e->a_pl, len, 16, 3, &st);
+  if (st) return st;
+  if (keep)
+    HIPCK(launch_relayout(e->p, (int)h.bead_off, h.n, i16 ? 1 : 0, h.pl_off, h.pl_cap, off, cap, e->stream));
+  const int64_t base = off;
+  const int32_t c32 = cap, zero = 0;
+  HIPCK(hipMemcpyAsync(&e->p.sys[slot].pl_base, &base, sizeof(base), hipMemcpyHostToDevice, e->stream));
+  HIPCK(hipMemcpyAsync(&e->p.sys[slot].pl_cap, &c32, sizeof(c32), hipMemcpyHostToDevice, e->stream));
+  if (!keep)  // no shell rows to filter: the next pair-list build comes from the cell grid
+    HIPCK(hipMemcpyAsync(&e->p.sys[slot].sl_valid, &zero, sizeof(zero), hipMemcpyHostToDevice, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  e->a_pl.release(h.pl_off, h.pl_len);
+  h.pl_off = off;
+  h.pl_len = len;
+  h.pl_cap = cap;
+  return SOP_OK;
+}
+
+// Systems whose longest row came within an eighth of their row capacity get
+// a larger one; their rows move and no result changes.  With fixed pools a
+// capacity that finds no room stays as it is (a row that outgrows it is
+// reported as an overflow).  The engine stream must be idle; sd is the
+// systems' current state.  *grown receives each system's added bytes.
+sop_status engine_grow_rows(sop_engine *e, const int32_t *slots, int32_t n, const std::vector<SysDev> &sd,
+                            std::vector<int64_t> *grown) {
+  if (grown) grown->assign((size_t)n, 0);
+  if (!e->grow_rows) return SOP_OK;
+  for (int32_t q = 0; q < n; ++q) {
+    const int32_t s = slots[q];
+    const SysHost &h = e->sys[s];
+    const int32_t m = sd[s].row_max;
+    if (!pl_cap_short(e, h, m)) continue;
+    const int64_t before = engine_system_bytes(e, s);
+    const sop_status st = engine_resize_rows(e, s, pl_cap_target(e, h.n, m), true);
+    if (st == SOP_ERR_CAPACITY && e->fixed_pools) continue;
+    if (st) return st;
+    if (grown) (*grown)[q] = engine_system_bytes(e, s) - before;
+  }
+  return SOP_OK;
+}
+
+// Before a system's shell widens to `shell`: a row capacity that the longest
+// row so far, scaled by the volume of the wider shell (and a tenth more),
+// would come within an eighth of grows now (rows moved, results unchanged),
+// so the first rebuild with the new shell does not overflow.
+sop_status engine_fit_shell(sop_engine *e, int32_t slot, float shell) {
+  SysHost &h = e->sys[slot];
+  if (!(shell > h.shell)) return SOP_OK;
+  SysDev S;
+  sop_status st = engine_read_sys(e, slot, &S);
+  if (st) return st;
+  float rc2 = 0.0f;
+  for (int q = 0; q < 4; ++q) rc2 = std::max(rc2, S.rc2_pl_rep[q]);
+  const double rc = std::sqrt((double)rc2);
+  const double ratio = (rc + shell) / (rc + h.shell);
+  const int32_t m = (int32_t)std::ceil((double)S.row_max * ratio * ratio * ratio * 1.1);
+  if (S.row_max <= 0 || !pl_cap_short(e, h, m)) return SOP_OK;
+  return engine_resize_rows(e, slot, pl_cap_target(e, h.n, m), true);
+}
+
+// After the first lists of newly added systems: a system whose longest row
+// overflowed its starting capacity or comes within an eighth of it gets the
+// fitted capacity, and its first lists are built again from the same
+// coordinates (the lists a fitted first build gives); its build counts are
+// set back to one build.  Capacities only grow, so a system never holds less
+// than job_bytes_estimate charged for it.
+sop_status engine_fit_new(sop_engine *e, const int32_t *slots, int32_t n) {
+  std::vector<SysDev> sd;
+  sop_status st = read_sys_all(e, sd);
+  if (st) return st;
+  std::vector<int32_t> redo;
+  for (int32_t q = 0; q < n; ++q) {
+    const int32_t s = slots[q];
+    const SysHost &h = e->sys[s];
+    const SysDev &S = sd[s];
+    if (S.err & ~ERR_PL_OVF) continue;  // failed for another reason: reported as it is
+    const int32_t m = S.row_max, want = pl_cap_target(e, h.n, m);
+    const bool over = (S.err & ERR_PL_OVF) != 0;
+    if (!over && !pl_cap_short(e, h, m)) continue;
+    if (want == h.pl_cap || want < m) continue;  // at the limit: an overflow stands
+    st = engine_resize_rows(e, s, want, false);
+    if (st) return st;
+    redo.push_back(s);
+  }
+  if (redo.empty()) return SOP_OK;
+  const int32_t zero = 0;
+  for (int32_t s : redo) {
+    HIPCK(hipMemcpy(&e->p.sys[s].err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(&e->p.sys[s].row_max, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  st = engine_set_batch(e, redo.data(), (int32_t)redo.size());
+  if (st) return st;
+  HIPCK(launch_lists(e->p, list_grids(e), 0, FORCE_NL | FORCE_PL, e->stream));
+  HIPCK(hipStreamSynchronize(e->stream));
+  for (int32_t s : redo)
+    HIPCK(hipMemcpy(e->p.sys[s].nbuild, sd[s].nbuild, sizeof(sd[s].nbuild), hipMemcpyHostToDevice));
+  return SOP_OK;
+}
+
 // First neighbor and pair lists of newly added systems, in one launch
-// sequence (the reference builds both before its step loop).  status[q]
-// receives each system's result; systems that fail are removed.
+// sequence (the reference builds both before its step loop), with row
+// capacities fitted.  status[q] receives each system's result; systems that
+// fail are removed.
 sop_status engine_init_lists(sop_engine *e, const int32_t *slots, int32_t n, sop_status *status) {
   if (n <= 0) return SOP_OK;
   sop_status st = engine_set_batch(e, slots, n);
   if (st) return st;
   HIPCK(launch_lists(e->p, list_grids(e), 0, FORCE_NL | FORCE_PL, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
+  st = engine_fit_new(e, slots, n);
+  if (st) return st;
   for (int32_t q = 0; q < n; ++q) {
     status[q] = engine_check_errors(e, &slots[q], 1);
     if (status[q]) engine_remove_system(e, slots[q]);
@@ -958,6 +1105,8 @@ static std::string graph_key(const sop_engine *e, int G, int n_nl, int n_pl) {
   return os.str();
 }
 
+constexpr int64_t GROW_BLOCKS = 20;
+
 // One hipGraph replay = G steps of an aligned batch (every system's step is a
 // multiple of n_nl): NL rebuild at k % n_nl == 0, PL filter at k % n_pl == 0,
 // fused step kernel every k, step-counter advance at the end.
@@ -1048,11 +1197,21 @@ sop_status engine_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t n
       mirror_advance(e, sys, nsys, pre);
       left -= pre;
     }
+    int64_t blocks = 0;
     while (left >= G) {
       st = engine_run_block_graph(e, (int32_t)G);
       if (st) return st;
       mirror_advance(e, sys, nsys, G);
       left -= G;
+      // row capacities checked every GROW_BLOCKS blocks of a long run
+      if (++blocks % GROW_BLOCKS == 0 && left >= G && !ev1) {
+        std::vector<SysDev> sd;
+        HIPCK(hipStreamSynchronize(e->stream));
+        st = read_sys_all(e, sd);
+        if (st) return st;
+        st = engine_grow_rows(e, sys, nsys, sd, nullptr);
+        if (st) return st;
+      }
     }
   }
   if (left > 0) {
@@ -1062,7 +1221,16 @@ sop_status engine_run(sop_engine *e, const int32_t *sys, int32_t nsys, int64_t n
   }
   if (ev1) HIPCK(hipEventRecord(ev1, e->stream));
   HIPCK(hipStreamSynchronize(e->stream));
-  return engine_check_errors(e, sys, nsys);
+  std::vector<SysDev> sd;
+  st = read_sys_all(e, sd);
+  if (st) return st;
+  st = engine_grow_rows(e, sys, nsys, sd, nullptr);
+  if (st) return st;
+  for (int32_t q = 0; q < nsys; ++q) {
+    st = engine_error_status(e, sys[q], sd[sys[q]].err);
+    if (st) return st;
+  }
+  return SOP_OK;
 }
 
 } // namespace sopk
@@ -1371,6 +1539,29 @@ sop_status sop_get_list(sop_engine *e, int32_t sys, int32_t which, int32_t *ij, 
     }
   *n = m;
   return SOP_OK;
+}
+
+sop_status sop_system_rows(sop_engine *e, int32_t sys, int32_t *capacity, int32_t *longest) {
+  if (!valid_sys(e, sys)) return set_error(e, SOP_ERR_INVALID, "no such system");
+  HIPCK(hipStreamSynchronize(e->stream));
+  SysDev S;
+  const sop_status st = engine_read_sys(e, sys, &S);
+  if (st) return st;
+  if (capacity) *capacity = e->sys[sys].pl_cap;
+  if (longest) *longest = S.row_max;
+  return SOP_OK;
+}
+
+sop_status sop_system_reserve_rows(sop_engine *e, int32_t sys, int32_t capacity) {
+  if (!valid_sys(e, sys) || capacity < 1) return set_error(e, SOP_ERR_INVALID, "bad arguments");
+  if (e->serve) return set_error(e, SOP_ERR_STATE, "engine is serving");
+  HIPCK(hipStreamSynchronize(e->stream));
+  SysDev S;
+  const sop_status st = engine_read_sys(e, sys, &S);
+  if (st) return st;
+  const SysHost &h = e->sys[sys];
+  const int32_t cap = std::min(pl_cap_limit(e, h.n), std::max(round_up(capacity, 16), round_up(S.row_max, 16)));
+  return engine_resize_rows(e, sys, cap, true);
 }
 
 sop_status sop_list_stats_get(sop_engine *e, int32_t sys, sop_list_stats *stt) {

@@ -25,6 +25,8 @@ hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg,
 hipError_t step_lds_limit(size_t bytes);
 hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st);
 hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st);
+hipError_t launch_relayout(const DevPtrs &p, int bo, int n, int idx16, int64_t old_base, int old_cap,
+                           int64_t new_base, int new_cap, hipStream_t st);
 hipError_t launch_begin(const DevPtrs &p, int64_t nsteps, hipStream_t st);
 hipError_t launch_selftest(int *out, float *fout, hipStream_t st);
 hipError_t noalloc_scratch(int device, void **ptr);
@@ -79,6 +81,7 @@ struct sop_engine {
   sop_engine_opts opts{};
   std::string err;
   bool fixed_pools = false;
+  bool grow_rows = true;  // row capacities grow after runs (off while the autotuner times trials)
 
   sopk::Pools cap;
   sopk::RangeAlloc a_beads, a_nat, a_pl, a_cells;
@@ -166,4 +169,9 @@ StepGrids step_grids(const sop_engine *e);
 StepGrids filter_grids(const sop_engine *e);
 ListGrids list_grids(const sop_engine *e);
 int64_t job_bytes_estimate(const sop_engine *e, const sop_system_desc *d);
+int64_t engine_system_bytes(const sop_engine *e, int32_t slot);
+sop_status engine_fit_new(sop_engine *e, const int32_t *slots, int32_t n);
+sop_status engine_fit_shell(sop_engine *e, int32_t slot, float shell);
+sop_status engine_grow_rows(sop_engine *e, const int32_t *slots, int32_t n, const std::vector<SysDev> &sd,
+                            std::vector<int64_t> *grown);
 } // namespace sopk

@@ -126,6 +126,13 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
+// Maximum over the wave's lanes (all lanes active).
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
 // Number of set bits of mask in lanes below this one (ballot prefix).
 __device__ __forceinline__ int lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
@@ -736,7 +743,7 @@ __global__ __launch_bounds__(256) void k_cellexpand(DevPtrs p) {
       __syncthreads();
       int o = base;
       for (int w = 0; w < wave; ++w) o += wsum[w];
-      if (used) p.cellitems[off + o + lanes_below(m)] = (q << 20) | c;
+      if (used) p.cellitems[off + o + lanes_below(m)] = make_int2(q, c);
       base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
       __syncthreads();
     }
@@ -847,9 +854,9 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int total = *p.ncellwork;
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
-    const int item = p.cellitems[g];
-    const int s = p.syslist[item >> 20];
-    const int c = item & 0xFFFFF;
+    const int2 item = p.cellitems[g];
+    const int s = p.syslist[item.x];
+    const int c = item.y;
     const SysDev &S = p.sys[s];
     const int bo = S.bead_off, cb = S.cell_base, n = S.n;
     const int pl_cap = S.pl_cap;
@@ -1017,6 +1024,11 @@ __global__ __launch_bounds__(256) void k_pl_cells(DevPtrs p, int kstep) {
           p.sys[s].sl_valid = 0;  // truncated rows must not be filtered again
         }
       }
+      if (wave == 0) {  // the longest row of this pass, for the host's capacity check
+        const int cr = lane < nh ? s_hcnt[lane] : 0;
+        const int mx = wave_max(cr);
+        if (lane == 0) atomicMax(&p.sys[s].row_max, mx);
+      }
       __syncthreads();
     }
   }
@@ -1107,6 +1119,7 @@ __global__ __launch_bounds__(256) void k_nl_build(DevPtrs p, int kstep) {
         if (lane == 0) {
           p.slrep_cnt[i] = min(cr, cap);
           if (cr > cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
+          atomicMax(&p.sys[s].row_max, cr);
         }
       } else {
         if (lane == 0) p.nlatt_cnt[i] = cnt;
@@ -1760,13 +1773,18 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
     if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL && S.build != BUILD_NL) continue;
     const float4 *pin = buf_in(p, S.parity + kstep);
     const int lds0 = KLDS ? it.z : 0;
+    // each wave owns stage_bytes / STEP_WAVES bytes whatever the entry width
+    // of the system it filters: the waves of a global-gather block move on to
+    // their next item without a barrier, so one wave may stage 16-bit rows
+    // while another stages 32-bit ones
+    char *wst = s_stage + wave * (stage_bytes / STEP_WAVES);
     const bool st16 = stage_off >= 0 && stage_bytes >= FILTER_STAGE_BYTES;
     const bool st32 = stage_off >= 0 && stage_bytes >= 2 * FILTER_STAGE_BYTES;
     if (S.idx16) {
-      if (st16) filter_tile<KLDS, true, true>(p, S, s, t, pin, s_pos, lds0, s_stage + wave * (FILTER_STAGE_BYTES / STEP_WAVES));
+      if (st16) filter_tile<KLDS, true, true>(p, S, s, t, pin, s_pos, lds0, wst);
       else filter_tile<KLDS, true, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
     } else {
-      if (st32) filter_tile<KLDS, false, true>(p, S, s, t, pin, s_pos, lds0, s_stage + wave * (2 * FILTER_STAGE_BYTES / STEP_WAVES));
+      if (st32) filter_tile<KLDS, false, true>(p, S, s, t, pin, s_pos, lds0, wst);
       else filter_tile<KLDS, false, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
     }
   }
@@ -1970,6 +1988,39 @@ __global__ void k_begin(DevPtrs p, int64_t nsteps) {
   }
 }
 
+// One system's shell rows and packed pair-list rows moved to a segment of
+// another row capacity (wave per tile): each row keeps its entries and its
+// place in the tile's order, so the lists, and every result computed from
+// them, are unchanged.  The two segments do not overlap.
+template <typename T>
+__device__ __forceinline__ void relayout_tile(const DevPtrs &p, int bo, int n, int t, int64_t ob, int oc, int64_t nb,
+                                              int nc) {
+  const int lane = threadIdx.x & 63;
+  const T *so = reinterpret_cast<const T *>(p.slrep_j + ob);
+  T *sn = reinterpret_cast<T *>(p.slrep_j + nb);
+  const T *po = reinterpret_cast<const T *>(p.plrep_j + ob);
+  T *pn = reinterpret_cast<T *>(p.plrep_j + nb);
+  const int shift = t * TILE * (nc - oc);  // the tile's region moves with the capacity
+  for (int q = 0; q < TILE; ++q) {
+    const int li = t * TILE + q;
+    if (li >= n) break;
+    const int i = bo + li;
+    const int cs = p.slrep_cnt[i];
+    for (int k = lane; k < cs; k += 64) sn[(int64_t)li * nc + k] = so[(int64_t)li * oc + k];
+    const int off = p.plrep_off[i];
+    const int len = (p.plrep_cnt[i] + ROW_PAD - 1) & ~(ROW_PAD - 1);
+    for (int k = lane; k < len; k += 64) pn[off + shift + k] = po[off + k];
+    if (lane == 0) p.plrep_off[i] = off + shift;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_relayout(DevPtrs p, int bo, int n, int idx16, int64_t ob, int oc, int64_t nb,
+                                                 int nc) {
+  const int t = blockIdx.x;
+  if (idx16) relayout_tile<uint16_t>(p, bo, n, t, ob, oc, nb, nc);
+  else relayout_tile<uint32_t>(p, bo, n, t, ob, oc, nb, nc);
+}
+
 // Per system: advance the step counter after a block of kernel steps.
 __global__ void k_advance(DevPtrs p, int nsteps) {
   const int nsys = *p.nsys;
@@ -2145,6 +2196,13 @@ hipError_t step_lds_limit(size_t bytes) {
     h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * FILTER_STAGE_BYTES));
   return h;
+}
+
+hipError_t launch_relayout(const DevPtrs &p, int bo, int n, int idx16, int64_t old_base, int old_cap,
+                           int64_t new_base, int new_cap, hipStream_t st) {
+  hipLaunchKernelGGL(k_relayout, dim3((n + TILE - 1) / TILE), dim3(64), 0, st, p, bo, n, idx16, old_base, old_cap,
+                     new_base, new_cap);
+  return SOPK_LAUNCH_CHECK();
 }
 
 hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st) {

@@ -104,6 +104,32 @@ sop_status retire_and_start(sop_engine *e, sop_engine::Serve *sv, const std::vec
   }
   st = hip_check(e, hipStreamSynchronize(e->stream), "retire sync");
   if (st) return st;
+  // row capacities: the admitted jobs' fitted to their first lists, the
+  // running ones' grown where their rows came close (the job's charge follows)
+  if (!added.empty()) {
+    std::vector<int64_t> before(added.size());
+    for (size_t q = 0; q < added.size(); ++q) before[q] = engine_system_bytes(e, added_slots[q]);
+    st = engine_fit_new(e, added_slots.data(), (int32_t)added_slots.size());
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t q = 0; q < added.size(); ++q) {
+      const int64_t d = engine_system_bytes(e, added_slots[q]) - before[q];
+      added[q]->bytes += d;
+      sv->used += d;
+    }
+  }
+  if (!keep.empty()) {
+    std::vector<int32_t> ks;
+    for (sop_engine::Job *j : keep) ks.push_back(j->slot);
+    std::vector<int64_t> grown;
+    st = engine_grow_rows(e, ks.data(), (int32_t)ks.size(), sd, &grown);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t q = 0; q < keep.size(); ++q) {
+      keep[q]->bytes += grown[q];
+      sv->used += grown[q];
+    }
+  }
   std::vector<SysDev> sa;
   if (!added.empty()) {
     sa.resize((size_t)e->cap.slots);
@@ -218,6 +244,9 @@ sop_status start_jobs(sop_engine *e, sop_engine::Serve *sv, const std::vector<so
       sv->used -= j->bytes;
       continue;
     }
+    const int64_t b = engine_system_bytes(e, j->slot);  // after its row capacity was fitted
+    sv->used += b - j->bytes;
+    j->bytes = b;
     sv->running.push_back(j);
   }
   return SOP_OK;

@@ -86,6 +86,7 @@ struct SysDev {
   int32_t type_mask; // bit t-1 set when some bead has type t
   // status, device-written
   int32_t err;
+  int32_t row_max;   // longest shell row (or neighbor-list build row) since the host last read it
 };
 
 // SysDev::build
@@ -145,7 +146,7 @@ struct DevPtrs {
   int32_t *ncellwork;   // total home cells of the direct pair-list build
   int32_t *nlbuild;     // 1 if a system of the batch materializes its neighbor list at this rebuild
   int32_t *gsort;       // nonzero if a grid build of this rebuild sorts in global memory (too big for LDS)
-  int32_t *cellitems;   // home cells of the direct build: (batch position << 20) | cell
+  int2 *cellitems;      // home cells of the direct build: {batch position, cell}
   int4 *items;          // step-kernel work: {first tile, tiles | waves per tile << 8, first staged
                         // bead or -1, staged beads}; the waves of a tile split its rounds
   int32_t *nitems;       // items that stage their systems in LDS: items[0 .. *nitems)

@@ -95,6 +95,10 @@ sop_status apply(sop_engine *e, const int32_t *sys, int32_t nsys, const Setting 
     SysHost &h = e->sys[sys[q]];
     const float w = engine_shell_for(e, h.n);
     if (w != h.shell) {
+      if (e->grow_rows) {  // room for the wider shell's rows (not inside timed trials)
+        const sop_status rs = engine_fit_shell(e, sys[q], w);
+        if (rs) return rs;
+      }
       h.shell = w;
       changed.push_back(sys[q]);
     }
@@ -162,6 +166,14 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
         break;
       }
   }
+  // row capacities with room for the widest shell tried, then no growth while
+  // trials run (a trial must not move rows the snapshot restores)
+  constexpr float SHELL_MAX_TRIED = 4.0f;
+  for (int32_t q = 0; q < nsys; ++q) {
+    const sop_status rs = engine_fit_shell(e, sys[q], SHELL_MAX_TRIED);
+    if (rs) return rs;
+  }
+  e->grow_rows = false;
   Snapshot snap;
   sop_status rc = snapshot_take(e, snap);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -205,7 +217,7 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
   }
   for (int b = 0; b < best.tune.nb; ++b) {
     if (!bucket_used[b]) continue;
-    for (float w : {2.0f, 3.0f, 4.0f}) {
+    for (float w : {2.0f, 3.0f, SHELL_MAX_TRIED}) {
       Setting c = best;
       c.tune.shell[b] = w;
       if (w != best.tune.shell[b]) consider(c);
@@ -227,9 +239,11 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
   // when a trial failed)
   if (snap.buf) {
     const sop_status r2 = snapshot_restore(e, snap);
+    e->grow_rows = true;
     const sop_status r3 = apply(e, sys, nsys, rc ? base : best);
     if (!rc) rc = r2 ? r2 : r3;
   }
+  e->grow_rows = true;
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (snap.buf) (void)hipFree(snap.buf);

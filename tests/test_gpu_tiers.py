@@ -67,3 +67,21 @@ def test_split_and_whole_tiles_are_bitwise_identical():
         for k in (0, 35, 71):
             assert np.array_equal(xa, b.coords(ib[k]))
         assert np.array_equal(a.force_eval(ia), b.force_eval(ib[5]))
+
+
+def test_global_gather_batch_of_16_and_32_bit_rows():
+    # systems above 65,536 beads keep 32-bit pair-list entries, smaller ones
+    # 16-bit: in one global-gather batch the shell filter's waves stage rows
+    # of both widths at once, each in its own part of the block's LDS
+    systems = [S.make_chain_system(41, 70000)] + [S.make_chain_system(42 + k, 2000 + 700 * k) for k in range(3)]
+    with S.Engine(0, lds_beads_max=-1) as e:
+        ids = [e.add(s) for s in systems]
+        print("mixed widths: 30 steps", flush=True)
+        e.run(ids, 30)
+        got = [e.coords(i) for i in ids]
+    with S.Engine(0, lds_beads_max=-1) as e:
+        for s, x in zip(systems, got):
+            i = e.add(s)
+            e.run([i], 30)
+            assert np.array_equal(e.coords(i), x), s.n
+            e.remove(i)
