@@ -768,7 +768,7 @@ constexpr int ROW_PAD = 8;
 // wave, 16 beads of a round x FILTER_SROW 16-bit entries (32-bit entries take
 // twice the bytes; longer rows are completed by a second pass)
 constexpr int LDS_BYTES_MAX = 160 * 1024;
-constexpr int FILTER_SROW = 128;
+constexpr int FILTER_SROW = 144;
 constexpr int FILTER_STAGE_BYTES = STEP_WAVES * 16 * FILTER_SROW * 2;
 // Buffer resource of `bytes` bytes at `base` (wave-uniform): loads through
 // it take a 32-bit offset, and a load past the end returns zeros.
@@ -1647,12 +1647,13 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     }
     // native pair list: natives within their pair cutoff, in native order,
     // packed per tile like the rows (platt_off): the keep decisions first,
-    // then the entries at the round's starts.  The first batch's decisions
-    // are kept as bits; beads with more natives than one batch (rare) have
-    // every batch decided again when the entries are written.
+    // kept as one bit per entry of the lane (up to 32), then the entries at
+    // the round's starts.  A batch past the first is loaded again for the
+    // writes; a bead with more than 128 natives (4 lanes x 32) has its
+    // decisions made again.
     if (!nlb) {
-      const bool multi = __ballot(nn > 4 * NB) != 0;
-      uint32_t km = 0;  // keep bits of the first batch
+      const bool multi = __ballot(nn > 4 * NB) != 0, huge = __ballot(nn > 4 * 32) != 0;
+      uint32_t km = 0;  // keep bits: bit m0 + b for the lane's entry l + 4 (m0 + b)
       auto nat_pass = [&](bool wr, int o0) -> int {
         int ncnt = 0;
         for (int m0 = 0; __ballot(l + 4 * m0 < nn) != 0; m0 += NB) {
@@ -1671,8 +1672,9 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             bool keep;
-            if (wr && !multi) {
-              keep = (km >> b) & 1u;
+            const int bit = m0 + b;
+            if (wr && !huge) {
+              keep = (km >> bit) & 1u;
             } else {
               const int k = l + 4 * (m0 + b);
               const float4 xj = partner<LDS>(pin, s_pos, lds0, nj[b]);
@@ -1680,7 +1682,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
               // the reference's predicate without the minimum image (see above)
               const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
               keep = (k < nn) & (((dx * dx + dy * dy) + dz * dz) < rc * rc);
-              if (!wr && m0 == 0 && keep) km |= 1u << b;
+              if (!wr && bit < 32 && keep) km |= 1u << bit;
             }
             const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
             if (wr && keep) p.platt[o0 + ncnt + __popc(bits & below)] = make_int2(nj[b], __float_as_int(nr0[b]));
