@@ -763,7 +763,14 @@ constexpr int EXCL_WORDS = CAND_MAX / 32;
 // ROW_PAD entries, so the step kernel reads whole chunks without a count
 // test: a self entry has d2 = 0, which the repulsion guard turns into zero
 // force.  pl_cap is a multiple of 16, so a padded row fits its capacity.
-constexpr int ROW_PAD = 8;
+constexpr int ROW_PAD = 4;
+// 16-bit pair-list entries per step-kernel chunk load (4: 8 bytes; 8: 16
+// bytes, which needs ROW_PAD 8): four entries per lane and load leave fewer
+// lanes idle at the end of a row than eight
+#ifndef STEP_CHUNK16
+#define STEP_CHUNK16 4
+#endif
+static_assert(ROW_PAD % STEP_CHUNK16 == 0 && ROW_PAD % 4 == 0, "rows must hold whole chunks");
 // LDS of one workgroup on gfx950, and the shell filter's row staging: per
 // wave, 16 beads of a round x FILTER_SROW 16-bit entries (32-bit entries take
 // twice the bytes; longer rows are completed by a second pass)
@@ -1216,10 +1223,20 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     t_br = p.bond_r0[b0 + lane];
   }
   const float br_before = b0 > bead_off ? p.bond_r0[b0 - 1] : 0.0f;  // bond (b0 - 1, b0)
-  // the non-native rows in 16-byte chunks: E = 8 16-bit or 4 32-bit entries
-  constexpr int E = IDX16 ? 8 : 4;
+  // the non-native rows in chunks of E entries: STEP_CHUNK16 16-bit entries
+  // (8 or 4: one 16- or 8-byte load) or 4 32-bit ones
+  constexpr int E = IDX16 ? STEP_CHUNK16 : 4;
   constexpr uint32_t ESZ = IDX16 ? 2u : 4u;
+  constexpr uint32_t CB = (uint32_t)E * ESZ;  // chunk bytes
   using chunk_t = uint4;
+  auto load_chunk = [&](__amdgpu_buffer_rsrc_t r, uint32_t off) -> chunk_t {
+    if constexpr (CB == 16) {
+      return load_b128(r, off);
+    } else {
+      const int2 v = load_i2(r, off);
+      return make_uint4((uint32_t)v.x, (uint32_t)v.y, 0u, 0u);
+    }
+  };
   // buffer resources in scalar registers: this system's pair-list segment
   // (rows packed per tile, plrep_off) and the native pair list, addressed by
   // 32-bit offsets
@@ -1232,9 +1249,6 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
     const int i = b0 + li;
     const int ns = __shfl(t_ns, li), cn = __shfl(t_cn, li), po = __shfl(t_po, li);
     const int nc = cn & 0xFFFF, nr = cn >> 16;
-    // bond r0 of (i, i + 1) and (i - 1, i), from the tile's values
-    const float br_up = __shfl(t_br, li), br_low_t = __shfl(t_br, (li + 63) & 63);
-    const float br_low = li > 0 ? br_low_t : br_before;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (i < bend) {
       // first loads of the native and non-native loops, issued before any
@@ -1243,41 +1257,13 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       const uint32_t rowoff = (uint32_t)po * ESZ;
       const int nch = (nr + E - 1) / E;
       chunk_t q = {};
-      if (l < nch) q = load_b128(rrow, rowoff + 16u * (uint32_t)l);
+      if (l < nch) q = load_chunk(rrow, rowoff + CB * (uint32_t)l);
       int2 na = make_int2(0, 0), nb2 = make_int2(0, 0);
       if (l < nc) na = load_i2(rnat, 8u * (uint32_t)(ns + l));
       if (l + LPB < nc) nb2 = load_i2(rnat, 8u * (uint32_t)(ns + l + LPB));
       const float4 xi = partner<LDS>(pin, s_pos, lds0, i);
       const int wi = __float_as_int(xi.w);
       const int row = ONE ? q1 : ((wi & 3) - 1) * 2;
-      // chain partners i-2, i-1, i+1, i+2: FENE bonds and i,i+2 soft sphere
-      if (l < 4) {
-        const int off = l < 2 ? l - 2 : l - 1;
-        const int j = i + off;
-        if (j >= bead_off && j < bead_off + n) {
-          const float4 xj = partner<LDS>(pin, s_pos, lds0, j);
-          if ((__float_as_int(xj.w) >> 2) == (wi >> 2)) {
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            float c;
-            if (off == 1 || off == -1) {
-              const float r0 = off < 0 ? br_low : br_up;
-              const float d = sqrtf(d2);
-              const float dev = d - r0;
-              const float gg = 1.0f - dev * dev * S.inv_R0sq;
-              c = S.k_bnd * dev * rcpf(d * gg);
-              if (gg <= 0.0f) atomicOr(&p.sys[s].err, ERR_BOND);
-            } else {
-              const float ir2 = rcpf(d2);
-              const float ir4 = ir2 * ir2;
-              c = -S.ss_f6 * (ir4 * ir4);
-            }
-            fx = fmaf(c, dx, fx);
-            fy = fmaf(c, dy, fy);
-            fz = fmaf(c, dz, fz);
-          }
-        }
-      }
       // native LJ over the native pair list (the first two entries per lane loaded above)
       {
         // c = -12 eps u q6 (q6 - 1) with u = 1/d2, q6 = (r0^2 u)^3, as
@@ -1330,11 +1316,13 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
         };
         for (int c = l; c < nch; c += LPB) {
           const chunk_t cur = q;
-          if (c + LPB < nch) q = load_b128(rrow, rowoff + 16u * (uint32_t)(c + LPB));
+          if (c + LPB < nch) q = load_chunk(rrow, rowoff + CB * (uint32_t)(c + LPB));
           uint32_t ww[E];
           if constexpr (IDX16) {
             ww[0] = cur.x & 0xFFFFu; ww[1] = cur.x >> 16; ww[2] = cur.y & 0xFFFFu; ww[3] = cur.y >> 16;
-            ww[4 % E] = cur.z & 0xFFFFu; ww[5 % E] = cur.z >> 16; ww[6 % E] = cur.w & 0xFFFFu; ww[7 % E] = cur.w >> 16;
+            if constexpr (E == 8) {
+              ww[4 % E] = cur.z & 0xFFFFu; ww[5 % E] = cur.z >> 16; ww[6 % E] = cur.w & 0xFFFFu; ww[7 % E] = cur.w >> 16;
+            }
           } else {
             ww[0] = cur.x; ww[1] = cur.y; ww[2] = cur.z; ww[3] = cur.w;
           }
@@ -1352,6 +1340,64 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
       mfy = vy;
       mfz = vz;
     }
+  }
+  // Chain terms, each pair once, over the tile's lanes: lane m computes the
+  // FENE bond (i, i+1) and the i,i+2 soft sphere of its bead i = b0 + m (the
+  // force on i; the partner takes the opposite), and takes the partner halves
+  // of the bond (i-1, i) and the pair (i-2, i) from lanes m-1 and m-2.  The
+  // pairs that start before the tile come from lanes 0 and 1 themselves.
+  {
+    const int i = b0 + lane;
+    const int send = bead_off + n;
+    float4 xc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < bend) xc = partner<LDS>(pin, s_pos, lds0, i);
+    // force on a from the pair (a, b) of chain offset off (1: FENE bond of
+    // length r0, 2: soft sphere); zero across chains
+    auto chain_pair = [&](float4 xa, int ja, int off, float r0, float &ox, float &oy, float &oz) {
+      ox = oy = oz = 0.0f;
+      const int jb = ja + off;
+      if (ja < bead_off || jb >= send) return;
+      const float4 xb = partner<LDS>(pin, s_pos, lds0, jb);
+      if ((__float_as_int(xb.w) >> 2) != (__float_as_int(xa.w) >> 2)) return;
+      const float dx = xb.x - xa.x, dy = xb.y - xa.y, dz = xb.z - xa.z;
+      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      float c;
+      if (off == 1) {
+        const float d = sqrtf(d2);
+        const float dev = d - r0;
+        const float gg = 1.0f - dev * dev * S.inv_R0sq;
+        c = S.k_bnd * dev * rcpf(d * gg);
+        if (gg <= 0.0f) atomicOr(&p.sys[s].err, ERR_BOND);
+      } else {
+        const float ir2 = rcpf(d2);
+        const float ir4 = ir2 * ir2;
+        c = -S.ss_f6 * (ir4 * ir4);
+      }
+      ox = c * dx;
+      oy = c * dy;
+      oz = c * dz;
+    };
+    float bx, by, bz, sx, sy, sz;
+    chain_pair(xc, i < bend ? i : send, 1, t_br, bx, by, bz);
+    chain_pair(xc, i < bend ? i : send, 2, 0.0f, sx, sy, sz);
+    // the partner halves from lanes m-1 (bond) and m-2 (soft sphere)
+    const int m1 = lane >= 1 ? lane - 1 : 0, m2 = lane >= 2 ? lane - 2 : 0;
+    float pbx = __shfl(bx, m1), pby = __shfl(by, m1), pbz = __shfl(bz, m1);
+    float psx = __shfl(sx, m2), psy = __shfl(sy, m2), psz = __shfl(sz, m2);
+    if (lane < 2 && i < bend) {  // pairs that start before the tile
+      float ex, ey, ez;
+      const float4 xp2 = i - 2 >= bead_off ? partner<LDS>(pin, s_pos, lds0, i - 2) : xc;
+      chain_pair(xp2, i - 2 >= bead_off ? i - 2 : send, 2, 0.0f, ex, ey, ez);
+      psx = ex; psy = ey; psz = ez;
+      if (lane == 0) {
+        const float4 xp1 = i - 1 >= bead_off ? partner<LDS>(pin, s_pos, lds0, i - 1) : xc;
+        chain_pair(xp1, i - 1 >= bead_off ? i - 1 : send, 1, br_before, ex, ey, ez);
+        pbx = ex; pby = ey; pbz = ez;
+      }
+    }
+    mfx += (bx + sx) - (pbx + psx);
+    mfy += (by + sy) - (pby + psy);
+    mfz += (bz + sz) - (pbz + psz);
   }
   const int i = b0 + lane;
   if (i >= bend) return;
