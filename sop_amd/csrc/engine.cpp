@@ -104,8 +104,14 @@ int grid_items(const sop_engine *e) {
 int grid_items_far(const sop_engine *e) { return items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nitems_far); }
 
 // Whole-CU LDS items: one block per compute unit (the block takes all of
-// the unit's LDS); never while serving.
-int grid_items_big(const sop_engine *e) { return items_grid(e->num_cu, false, e->cur_nitems_big); }
+// the unit's LDS).  While serving, a fixed grid whenever the batch has such
+// items and no launch when it has none (two graph variants; a light load of
+// small jobs then pays for no empty launch).
+static int items_grid_big(const sop_engine *e, int count) {
+  if (e->fixed_grids) return count > 0 ? items_grid(e->num_cu, true, count) : 0;
+  return items_grid(e->num_cu, false, count);
+}
+int grid_items_big(const sop_engine *e) { return items_grid_big(e, e->cur_nitems_big); }
 
 StepGrids step_grids(const sop_engine *e) {
   return StepGrids{grid_items(e), grid_items_big(e), grid_items_far(e), e->cur_lds, e->cur_lds_big};
@@ -113,7 +119,7 @@ StepGrids step_grids(const sop_engine *e) {
 
 StepGrids filter_grids(const sop_engine *e) {
   return StepGrids{items_grid(e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu, e->fixed_grids, e->cur_nfitems),
-                   items_grid(e->num_cu, false, e->cur_nfitems_big),
+                   items_grid_big(e, e->cur_nfitems_big),
                    items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nfitems_far), e->cur_flds, e->cur_flds_big};
 }
 
@@ -836,11 +842,9 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
                               ? std::min<int64_t>(e->opts.lds_beads_max, (int64_t)(e->lds_limit / 16))
                               : 0;
   // systems above lds_cap that fit one compute unit's whole LDS take the
-  // whole-CU tier (one block per unit) instead of global gathers.  Not while
-  // serving: there every tier is launched at every step (fixed grids), and
-  // each launch costs at least one tile's latency in series with the others,
-  // which measured more than the tier saves on a served job mix.
-  const int64_t big_cap = e->fixed_grids ? 0 : e->lds_big_cap;
+  // whole-CU tier (one block per unit) instead of global gathers, also while
+  // serving (the served stream's capacity 3.8e9 -> 4.5e9 bead-steps/s)
+  const int64_t big_cap = e->lds_big_cap;
   std::vector<int32_t> by_arena(slots, slots + nslots);
   std::sort(by_arena.begin(), by_arena.end(),
             [&](int32_t a2, int32_t b2) { return e->sys[a2].bead_off < e->sys[b2].bead_off; });
@@ -990,7 +994,10 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
     out.n_lds = (int32_t)(imx * NX);
     out.n_big = (int32_t)(bmx * NX);
     out.n_far = (int32_t)(fmx * NX);
+    // fixed grids (serving): the launches' LDS sizes too, so that one captured
+    // graph serves every batch
     if (e->fixed_grids && lds_cap > 0) out.lds = (size_t)lds_cap * 16;
+    if (e->fixed_grids && big_cap > 0) out.lds_big = (size_t)big_cap * 16;
   };
   Plan sp, fp;
   plan(0, true, sp);
