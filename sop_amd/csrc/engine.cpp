@@ -841,10 +841,6 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   const int64_t lds_cap = e->opts.lds_beads_max > 0
                               ? std::min<int64_t>(e->opts.lds_beads_max, (int64_t)(e->lds_limit / 16))
                               : 0;
-  // systems above lds_cap that fit one compute unit's whole LDS take the
-  // whole-CU tier (one block per unit) instead of global gathers, also while
-  // serving (the served stream's capacity 3.8e9 -> 4.5e9 bead-steps/s)
-  const int64_t big_cap = e->lds_big_cap;
   std::vector<int32_t> by_arena(slots, slots + nslots);
   std::sort(by_arena.begin(), by_arena.end(),
             [&](int32_t a2, int32_t b2) { return e->sys[a2].bead_off < e->sys[b2].bead_off; });
@@ -854,6 +850,14 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
   int64_t total_tiles = 0;
   for (int32_t q = 0; q < nslots; ++q) total_tiles += e->sys[slots[q]].bead_len / TILE;
   const int64_t slots_w = (int64_t)e->num_cu * STEP_WAVES;
+  // systems above lds_cap that fit one compute unit's whole LDS take the
+  // whole-CU tier (one block per unit) instead of global gathers.  While
+  // serving only under load (more tiles than two LDS launches' wave slots):
+  // there it raises the stream's capacity (3.8e9 -> 4.9e9 bead-steps/s at its
+  // stated parameters), while under a light load the global-gather tier,
+  // spread over more blocks, gives the shorter step (served p50 0.25 s
+  // against 0.30 s)
+  const int64_t big_cap = e->fixed_grids && total_tiles <= 2 * slots_w ? 0 : e->lds_big_cap;
   const int lpb = e->opts.lanes_per_bead;
   int32_t per_step = e->opts.step_tiles;
   if (per_step <= 0) per_step = total_tiles > slots_w ? STEP_WAVES : (2 * total_tiles > slots_w ? 8 : 4);
