@@ -153,3 +153,34 @@ def test_schedule_mismatch_is_refused():
         with pytest.raises(S.SopError):
             srv.submit(c, 10)
         srv.stop()
+
+
+def test_loaded_server_whole_cu_tier_equals_alone():
+    # more tiles than two LDS launches' wave slots (a loaded server): jobs of
+    # 5,057-10,112 beads take the whole-CU LDS tier beside the two-blocks-per-
+    # unit one; every result still equals the job run alone
+    rng = np.random.default_rng(12)
+    sizes = [int(v) for v in rng.integers(1500, 3000, 320)] + [6000, 7500, 9000, 12000]
+    systems = []
+    for k, n in enumerate(sizes):
+        s = S.make_chain_system(900 + k % 40, n)
+        s.params.seed = W.philox_key(13, k)
+        s.params.T = 0.55 + 0.001 * k
+        systems.append(s)
+    steps = 200
+    with S.Engine(0) as eng:
+        srv = serving.Server(eng, hbm_budget_bytes=64 << 30)
+        try:
+            ids = [srv.submit(s, steps) for s in systems]
+            res = [srv.wait(j) for j in ids]
+        finally:
+            srv.stop()
+    assert all(r["status"] == 0 for r, _ in res)
+    check = [0, 150, len(sizes) - 4, len(sizes) - 3, len(sizes) - 2, len(sizes) - 1]
+    with S.Engine(0) as e:
+        for k in check:
+            i = e.add(systems[k])
+            e.run([i], steps)
+            assert np.array_equal(res[k][1], e.coords(i)), (k, sizes[k])
+            assert np.array_equal(res[k][0]["energies"], e.energy_eval(i)[:6]), k
+            e.remove(i)
