@@ -1557,7 +1557,7 @@ __device__ __forceinline__ void step_items(const DevPtrs &p, int kstep, float4 *
 // known; the entries past FILTER_SROW, and every entry without staging, come
 // from a second pass over the round's rows, which repeats the first's
 // compaction order.  Rows are padded to ROW_PAD entries for the step kernel.
-template <bool LDS, bool IDX16, bool STAGE>
+template <bool LDS, bool IDX16, bool STAGE, bool ONE>
 __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, int s, int t,
                                             const float4 *__restrict__ pin, const float4 *s_pos, int lds0,
                                             void *s_stage) {
@@ -1631,7 +1631,12 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     }
     const float4 xi = partner<LDS>(pin, s_pos, lds0, self);
     const bool t2 = (__float_as_int(xi.w) & 3) == 2;
-    const float rcA = t2 ? rc21 : rc11, rcB = t2 ? rc22 : rc12;
+    // the cutoffs by the partner's type (one-type systems: the one of their type)
+    const float rcA = ONE ? (t2 ? rc22 : rc11) : (t2 ? rc21 : rc11), rcB = t2 ? rc22 : rc12;
+    // entry value of the bead itself, and entry value -> staged position (LDS)
+    // or arena index
+    const int vself = IDX16 ? self - bo : self;
+    const int vpos = LDS ? (IDX16 ? bo - lds0 : -lds0) : (IDX16 ? bo : 0);
     const chunk_t *src = reinterpret_cast<const chunk_t *>(
         reinterpret_cast<const ent_t *>(p.slrep_j + pl_base) + (int64_t)(self - bo) * pl_cap);
     ent_t *s_row = STAGE ? reinterpret_cast<ent_t *>(s_stage) + g * FILTER_SROW : nullptr;  // this group's staged row
@@ -1653,19 +1658,22 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
         for (int b = 0; b < CB; ++b) {
           const int c = l + 4 * (m0 + b);
           if (__ballot(c < nch) == 0) break;
-          int jj[4];
+          // entry values as stored (16-bit local or 32-bit arena indices);
+          // an entry past the row's end reads the bead itself
+          int v[4];
           if constexpr (IDX16) {
-            jj[0] = bo + (int)(ch[b].x & 0xFFFFu); jj[1] = bo + (int)(ch[b].x >> 16);
-            jj[2] = bo + (int)(ch[b].y & 0xFFFFu); jj[3] = bo + (int)(ch[b].y >> 16);
+            v[0] = (int)(ch[b].x & 0xFFFFu); v[1] = (int)(ch[b].x >> 16);
+            v[2] = (int)(ch[b].y & 0xFFFFu); v[3] = (int)(ch[b].y >> 16);
           } else {
-            jj[0] = (int)ch[b].x; jj[1] = (int)ch[b].y; jj[2] = (int)ch[b].z; jj[3] = (int)ch[b].w;
+            v[0] = (int)ch[b].x; v[1] = (int)ch[b].y; v[2] = (int)ch[b].z; v[3] = (int)ch[b].w;
           }
           bool ok[4];
           float4 xj[4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             ok[e] = 4 * c + e < m;
-            xj[e] = partner<LDS>(pin, s_pos, lds0, ok[e] ? jj[e] : self);
+            const int q = (ok[e] ? v[e] : vself) + vpos;
+            xj[e] = LDS ? s_pos[q] : pin[q];
           }
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
@@ -1673,9 +1681,10 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
             // identity here (a grid build whose extent reaches boxl/2 - margin
             // raises ERR_PBC, and beads move less than the margin before the next)
             const float dx = xj[e].x - xi.x, dy = xj[e].y - xi.y, dz = xj[e].z - xi.z;
-            const bool keep = ok[e] & (nlb | (((dx * dx + dy * dy) + dz * dz) < (w_type(xj[e].w) == 2 ? rcB : rcA)));
-            const uint32_t bits = (uint32_t)(__ballot(keep) >> (4 * g)) & 0xFu;
-            if (keep) put(cnt + __popc(bits & below), jj[e]);
+            const float rc = ONE ? rcA : (w_type(xj[e].w) == 2 ? rcB : rcA);
+            const bool keep = ok[e] & (nlb | (((dx * dx + dy * dy) + dz * dz) < rc));
+            const uint32_t bits = (uint32_t)(__builtin_amdgcn_ballot_w64(keep) >> (4 * g)) & 0xFu;
+            if (keep) put(cnt + __popc(bits & below), v[e]);
             cnt += __popc(bits);
           }
         }
@@ -1685,8 +1694,8 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     const int mrow = act ? nr : 0;
     int cnt;
     if constexpr (STAGE) {
-      cnt = pass(mrow, [&](int o, int j) {
-        if (o < FILTER_SROW) s_row[o] = (ent_t)(IDX16 ? j - bo : j);
+      cnt = pass(mrow, [&](int o, int v) {
+        if (o < FILTER_SROW) s_row[o] = (ent_t)v;
       });
     } else {
       cnt = pass(mrow, [](int, int) {});
@@ -1784,8 +1793,8 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     // the entries not staged: a second pass over the rows that have them
     const int lo = STAGE ? FILTER_SROW : 0;
     if (__ballot(c > lo) != 0) {
-      pass(c > lo ? mrow : 0, [&](int o, int j) {
-        if (o >= lo && o < c) dst[o] = (ent_t)(IDX16 ? j - bo : j);
+      pass(c > lo ? mrow : 0, [&](int o, int v) {
+        if (o >= lo && o < c) dst[o] = (ent_t)v;
       });
     }
   }
@@ -1828,12 +1837,18 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
     char *wst = s_stage + wave * (stage_bytes / STEP_WAVES);
     const bool st16 = stage_off >= 0 && stage_bytes >= FILTER_STAGE_BYTES;
     const bool st32 = stage_off >= 0 && stage_bytes >= 2 * FILTER_STAGE_BYTES;
+    // one-type systems: every partner has the bead's type (one cutoff)
+    const bool one = S.type_mask != 3;
     if (S.idx16) {
-      if (st16) filter_tile<KLDS, true, true>(p, S, s, t, pin, s_pos, lds0, wst);
-      else filter_tile<KLDS, true, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
+      if (st16) {
+        if (one) filter_tile<KLDS, true, true, true>(p, S, s, t, pin, s_pos, lds0, wst);
+        else filter_tile<KLDS, true, true, false>(p, S, s, t, pin, s_pos, lds0, wst);
+      } else {
+        filter_tile<KLDS, true, false, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
+      }
     } else {
-      if (st32) filter_tile<KLDS, false, true>(p, S, s, t, pin, s_pos, lds0, wst);
-      else filter_tile<KLDS, false, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
+      if (st32) filter_tile<KLDS, false, true, false>(p, S, s, t, pin, s_pos, lds0, wst);
+      else filter_tile<KLDS, false, false, false>(p, S, s, t, pin, s_pos, lds0, nullptr);
     }
   }
 }
