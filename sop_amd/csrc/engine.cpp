@@ -10,6 +10,8 @@
 #include <cstring>
 #include <numeric>
 #include <sstream>
+#include <unordered_map>
+#include <utility>
 
 using namespace sopk;
 
@@ -957,17 +959,35 @@ sop_status engine_set_batch(sop_engine *e, const int32_t *slots, int32_t nslots)
         lds_items.swap(cut);
       }
     }
-    for (const int4 &it : lds_items) {
-      const int x = (int)(std::min_element(iload.begin(), iload.end()) - iload.begin());
-      ibins[x].push_back(it);
-      iload[x] += (double)item_tiles(it.y);
-    }
+    // the items that stage one range (one system's items) go to one XCD, so
+    // that their stagings after the first read the range from that XCD's L2;
+    // ranges by load, largest first, each to the least-loaded XCD, then each
+    // XCD's items longest first
+    auto bin_by_range = [&](const std::vector<int4> &items, std::vector<std::vector<int4>> &bins,
+                            std::vector<double> &load) {
+      std::vector<std::pair<double, std::vector<int4>>> ranges;
+      std::unordered_map<int32_t, size_t> at;
+      for (const int4 &it : items) {
+        auto f = at.find(it.z);
+        if (f == at.end()) {
+          f = at.emplace(it.z, ranges.size()).first;
+          ranges.emplace_back(0.0, std::vector<int4>());
+        }
+        ranges[f->second].first += (double)item_tiles(it.y);
+        ranges[f->second].second.push_back(it);
+      }
+      std::stable_sort(ranges.begin(), ranges.end(),
+                       [](const auto &u, const auto &v) { return u.first > v.first; });
+      for (const auto &r : ranges) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        bins[x].insert(bins[x].end(), r.second.begin(), r.second.end());
+        load[x] += r.first;
+      }
+      for (auto &bn : bins) std::stable_sort(bn.begin(), bn.end(), by_dur);
+    };
+    bin_by_range(lds_items, ibins, iload);
     std::stable_sort(big_items.begin(), big_items.end(), by_dur);
-    for (const int4 &it : big_items) {
-      const int x = (int)(std::min_element(bload.begin(), bload.end()) - bload.begin());
-      bbins[x].push_back(it);
-      bload[x] += (double)item_tiles(it.y);
-    }
+    bin_by_range(big_items, bbins, bload);
     std::sort(far_items.begin(), far_items.end(), [](const std::vector<int4> &u, const std::vector<int4> &v) {
       return u.size() != v.size() ? u.size() > v.size() : u[0].x < v[0].x;
     });
