@@ -1314,9 +1314,12 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
           fy = fmaf(c, dy, fy);
           fz = fmaf(c, dz, fz);
         };
+#pragma unroll 2
         for (int c = l; c < nch; c += LPB) {
           const chunk_t cur = q;
-          if (c + LPB < nch) q = load_chunk(rrow, rowoff + CB * (uint32_t)(c + LPB));
+          // the next chunk's load without a condition: past the row it reads
+          // the rows after it, or zeros past the segment (buffer range check)
+          q = load_chunk(rrow, rowoff + CB * (uint32_t)(c + LPB));
           uint32_t ww[E];
           if constexpr (IDX16) {
             ww[0] = cur.x & 0xFFFFu; ww[1] = cur.x >> 16; ww[2] = cur.y & 0xFFFFu; ww[3] = cur.y >> 16;
