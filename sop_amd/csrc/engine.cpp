@@ -1345,6 +1345,12 @@ sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_en
     (void)hipGetLastError();
     e->lds_limit = 65536 - 2048;
   }
+  // the shell filter's launches need their LDS sizes: no fallback
+  if (filter_lds_init() != hipSuccess) {
+    (void)hipGetLastError();
+    delete e;
+    return SOP_ERR_HIP;
+  }
   {
     const int64_t whole = (int64_t)(e->lds_limit / 16) / TILE * TILE;
     e->lds_big_cap = e->opts.lds_beads_max > 0 && whole > e->opts.lds_beads_max ? whole : 0;

@@ -23,6 +23,7 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
 hipError_t launch_step(const DevPtrs &p, int mode, int lpb, const StepGrids &sg, int kstep, hipStream_t st,
                        const StepFork *fk = nullptr);
 hipError_t step_lds_limit(size_t bytes);
+hipError_t filter_lds_init();
 hipError_t launch_energy(const DevPtrs &p, int grid_tiles, int grid_sys, hipStream_t st);
 hipError_t launch_advance(const DevPtrs &p, int nsteps, hipStream_t st);
 hipError_t launch_relayout(const DevPtrs &p, int bo, int n, int idx16, int64_t old_base, int old_cap,

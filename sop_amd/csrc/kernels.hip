@@ -1810,7 +1810,7 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
 // stage_bytes (16-bit entries need FILTER_STAGE_BYTES, 32-bit twice that).
 template <bool KLDS>
 __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int kstep, int tier, int stage_off,
-                                                               int stage_bytes) {
+                                                               int stage_bytes, int builds) {
   extern __shared__ float4 s_pos[];
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1820,17 +1820,25 @@ __global__ __launch_bounds__(STEP_THREADS) void k_shell_filter(DevPtrs p, int ks
   for (int w = w0 + blockIdx.x; w < w1; w += gridDim.x) {
     const int4 it = p.fitems[w];
     if (it.x < 0) continue;
+    // one wave per tile (the tile order it computes first must come from one
+    // wave); a tile is filtered when its system's build is one of `builds`
+    // (bit per build kind).  Every wave reads the whole item's decisions, so
+    // an item with nothing to filter is skipped by the whole block (no
+    // staging) without a block-wide vote.
+    const int lane = tid & 63;
+    bool mine = false;
+    if (lane < item_tiles(it.y)) mine = ((builds >> p.sys[p.tile_sys[it.x + lane]].build) & 1) != 0;
+    const uint64_t tiles_mine = __ballot(mine);
+    if (tiles_mine == 0) continue;
     if (KLDS) {
-      __syncthreads();
-      stage_item(p, it.z, it.w, kstep, s_pos, wave, tid & 63);
+      __syncthreads();  // the previous item's reads are done
+      stage_item(p, it.z, it.w, kstep, s_pos, wave, lane);
       __syncthreads();
     }
-    // one wave per tile (the tile order it computes first must come from one wave)
-    if (wave >= item_tiles(it.y)) continue;
+    if (((tiles_mine >> wave) & 1) == 0) continue;
     const int t = it.x + wave;
     const int s = __builtin_amdgcn_readfirstlane(p.tile_sys[t]);
     const SysDev &S = p.sys[s];
-    if (S.build != BUILD_DIRECT && S.build != BUILD_SHELL && S.build != BUILD_NL) continue;
     const float4 *pin = buf_in(p, S.parity + kstep);
     const int lds0 = KLDS ? it.z : 0;
     // each wave owns stage_bytes / STEP_WAVES bytes whatever the entry width
@@ -2159,6 +2167,24 @@ static hipError_t fork_end(const StepFork *fk, hipStream_t st) {
   return h;
 }
 
+// The shell filter's launches (LDS items, whole-CU items, global-gather
+// items) on one stream, for the systems whose build is one of `builds`.
+static void launch_filter(const DevPtrs &p, const StepGrids &sg, int kstep, int builds, hipStream_t st) {
+  // row staging after the staged positions where LDS has room for it
+  const bool stage0 = sg.lds + FILTER_STAGE_BYTES <= LDS_BYTES_MAX;
+  if (sg.grid > 0)
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS),
+                       sg.lds + (stage0 ? FILTER_STAGE_BYTES : 0), st, p, kstep, 0, stage0 ? (int)sg.lds : -1,
+                       stage0 ? FILTER_STAGE_BYTES : 0, builds);
+  if (sg.grid_big > 0)
+    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, st, p, kstep, 1,
+                       -1, 0, builds);
+  // global-gather items stage no coordinates: room for 32-bit rows
+  if (sg.grid_far > 0)
+    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 2 * FILTER_STAGE_BYTES, st,
+                       p, kstep, 0, 0, 2 * FILTER_STAGE_BYTES, builds);
+}
+
 // One list-rebuild point for the current batch: snapshot / decision, cell
 // grid and counting sort, then whichever of the direct pair-list build or the
 // neighbor-list materialization + filter each system needs.
@@ -2167,40 +2193,36 @@ hipError_t launch_lists(const DevPtrs &p, const ListGrids &lg, int kstep, int fl
   const int grid_tiles = lg.grid_tiles, grid_sys = lg.grid_sys;
   hipLaunchKernelGGL(k_skin, dim3(grid_sys), dim3(SKIN_THREADS), 0, st, p, kstep, flags);
   hipLaunchKernelGGL(k_bbox, dim3(grid_sys), dim3(256), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_cell_sort_lds, dim3(grid_sys), dim3(SORT_THREADS), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_cell_scan, dim3(grid_sys), dim3(256), 0, st, p);
-  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_tiles), dim3(64), 0, st, p);
-  hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_cellwork, dim3(1), dim3(256), 0, st, p);
-  hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, st, p);
-  hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
-  hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, st, p);
-  // neighbor-list builds: their rows before the shell filter packs them
-  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, st, p, kstep);
-  const StepGrids &sg = lg.fg;
-  const bool fork = fork_wanted(fk, sg);
-  if (fork) {
+  // Systems whose pair list comes from their shell rows need nothing from
+  // the cell grid: with a second stream (fk) their filter runs beside the
+  // grid chain, and the filter of the grid-built systems follows the chain on
+  // the second stream.  The two filters take disjoint systems.
+  constexpr int ALL = (1 << BUILD_DIRECT) | (1 << BUILD_SHELL) | (1 << BUILD_NL);
+  constexpr int SHELL_ONLY = 1 << BUILD_SHELL;
+  hipStream_t cs = st;  // the grid chain's stream
+  if (fk) {
     const hipError_t h = fork_begin(fk, st);
     if (h != hipSuccess) return h;
+    cs = fk->side;
   }
-  const hipStream_t side = fork ? fk->side : st;
-  // row staging after the staged positions where LDS has room for it
-  const bool stage0 = sg.lds + FILTER_STAGE_BYTES <= LDS_BYTES_MAX;
-  if (sg.grid > 0)
-    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid), dim3(STEP_THREADS),
-                       sg.lds + (stage0 ? FILTER_STAGE_BYTES : 0), st, p, kstep, 0, stage0 ? (int)sg.lds : -1,
-                       stage0 ? FILTER_STAGE_BYTES : 0);
-  if (sg.grid_big > 0)
-    hipLaunchKernelGGL(k_shell_filter<true>, dim3(sg.grid_big), dim3(STEP_THREADS), sg.lds_big, side, p, kstep, 1,
-                       -1, 0);
-  // global-gather items stage no coordinates: room for 32-bit rows
-  if (sg.grid_far > 0)
-    hipLaunchKernelGGL(k_shell_filter<false>, dim3(sg.grid_far), dim3(STEP_THREADS), 2 * FILTER_STAGE_BYTES, side,
-                       p, kstep, 0, 0, 2 * FILTER_STAGE_BYTES);
-  if (fork) {
+  hipLaunchKernelGGL(k_cell_sort_lds, dim3(grid_sys), dim3(SORT_THREADS), 0, cs, p, kstep);
+  hipLaunchKernelGGL(k_cell_count, dim3(grid_tiles), dim3(64), 0, cs, p, kstep);
+  hipLaunchKernelGGL(k_cell_scan, dim3(grid_sys), dim3(256), 0, cs, p);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_tiles), dim3(64), 0, cs, p);
+  hipLaunchKernelGGL(k_cell_rank, dim3(grid_tiles), dim3(64), 0, cs, p, kstep);
+  hipLaunchKernelGGL(k_cellwork, dim3(1), dim3(256), 0, cs, p);
+  hipLaunchKernelGGL(k_cellexpand, dim3(grid_sys), dim3(256), 0, cs, p);
+  hipLaunchKernelGGL(k_pl_cells, dim3(grid_tiles), dim3(256), 0, cs, p, kstep);
+  hipLaunchKernelGGL(k_tile_order, dim3(grid_tiles), dim3(256), 0, cs, p);
+  // neighbor-list builds: their rows before the shell filter packs them
+  hipLaunchKernelGGL(k_nl_build, dim3(grid_tiles), dim3(256), 0, cs, p, kstep);
+  if (fk) {
+    launch_filter(p, lg.fg, kstep, ALL & ~SHELL_ONLY, cs);
+    launch_filter(p, lg.fg, kstep, SHELL_ONLY, st);
     const hipError_t h = fork_end(fk, st);
     if (h != hipSuccess) return h;
+  } else {
+    launch_filter(p, lg.fg, kstep, ALL, st);
   }
   return SOPK_LAUNCH_CHECK();
 }
@@ -2255,9 +2277,14 @@ hipError_t step_lds_limit(size_t bytes) {
   SOPK_LDS_ATTR(4, 1) SOPK_LDS_ATTR(8, 1) SOPK_LDS_ATTR(16, 1) SOPK_LDS_ATTR(32, 1)
   SOPK_LDS_ATTR(4, 2) SOPK_LDS_ATTR(8, 2) SOPK_LDS_ATTR(16, 2) SOPK_LDS_ATTR(32, 2)
 #undef SOPK_LDS_ATTR
-  if (h == hipSuccess)
-    h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<true>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES_MAX);
+  return h;
+}
+
+// Allow the shell filter the dynamic LDS its launches take (a compute unit's
+// whole LDS for the LDS items, row staging for the global-gather ones).
+hipError_t filter_lds_init() {
+  hipError_t h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES_MAX);
   if (h == hipSuccess)
     h = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shell_filter<false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * FILTER_STAGE_BYTES));
