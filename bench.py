@@ -219,9 +219,19 @@ def traffic_probe(child_args, kernel_prefix):
     f_write = nb / (cw["WRITE_SIZE"] * 1024.0)
     read = kf["FETCH_SIZE"] * 1024.0 * f_fetch
     write = kw["WRITE_SIZE"] * 1024.0 * f_write
-    return {"bytes": read + write, "read": read, "write": write, "fetch_scale": f_fetch, "write_scale": f_write,
-            "valu_insts": kf.get("SQ_INSTS_VALU"), "lds_insts": kf.get("SQ_INSTS_LDS"), "waves": kf.get("SQ_WAVES"),
-            "dispatches": kf["dispatches"]}
+    out = {"bytes": read + write, "read": read, "write": write, "fetch_scale": f_fetch, "write_scale": f_write,
+           "valu_insts": kf.get("SQ_INSTS_VALU"), "lds_insts": kf.get("SQ_INSTS_LDS"), "waves": kf.get("SQ_WAVES"),
+           "dispatches": kf["dispatches"]}
+    # the read requests by size: FETCH_SIZE counts a 128-byte request as 128
+    # bytes only through TCC_BUBBLE, which reads 0 on this device, so the
+    # request counts give the reads independently of the copy's scale
+    kr = pmc_probe(["TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"], child_args,
+                   kernel_prefix)
+    if kr:
+        n_all, n64, n128 = kr["TCC_EA0_RDREQ_sum"], kr["TCC_EA0_RDREQ_64B_sum"], kr["TCC_EA0_RDREQ_128B_sum"]
+        out["read_requests"] = {"all": n_all, "64B": n64, "128B": n128,
+                                "bytes": 32.0 * max(0.0, n_all - n64 - n128) + 64.0 * n64 + 128.0 * n128}
+    return out
 
 
 def main():
@@ -387,6 +397,10 @@ def main():
         if traffic:
             log(f"step kernel HBM traffic {traffic['bytes'] / 1e6:.1f} MB/launch "
                 f"({traffic['bytes'] / step_bytes:.2f}x algorithmic), {traffic['valu_insts']:.4g} VALU instructions")
+            rq = traffic.get("read_requests")
+            if rq:
+                log(f"step kernel reads by request size: {rq['128B']:.4g} x 128 B + {rq['64B']:.4g} x 64 B "
+                    f"= {rq['bytes'] / 1e6:.1f} MB (FETCH_SIZE scaled: {traffic['read'] / 1e6:.1f} MB)")
 
     cpu = None
     if not args.no_cpu:

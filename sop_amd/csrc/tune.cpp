@@ -186,15 +186,26 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
   if (!rc) rc = trial(e, sys, nsys, snap, base, G, blocks, ev0, ev1, &best_ms);
   base_ms = best_ms;
   ++trials;
-  // one parameter at a time, keeping a change only when it gains more than 1 %
+  // one parameter at a time, keeping a change only when it gains more than
+  // 1 % twice: a candidate that wins its trial is timed again beside the
+  // setting it would replace, so that one lucky trial (clock or placement
+  // noise of a few per cent) does not decide
   auto consider = [&](const Setting &cand) {
     if (rc) return;
     double ms = 0.0;
     rc = trial(e, sys, nsys, snap, cand, G, blocks, ev0, ev1, &ms);
     ++trials;
-    if (!rc && ms < 0.99 * best_ms) {
-      best_ms = ms;
+    if (rc || !(ms < 0.99 * best_ms)) return;
+    double ms_inc = 0.0, ms2 = 0.0;
+    rc = trial(e, sys, nsys, snap, best, G, blocks, ev0, ev1, &ms_inc);
+    if (!rc) rc = trial(e, sys, nsys, snap, cand, G, blocks, ev0, ev1, &ms2);
+    trials += 2;
+    if (rc) return;
+    if (ms2 < 0.99 * ms_inc) {
+      best_ms = std::min(ms, ms2);
       best = cand;
+    } else {
+      best_ms = std::min(best_ms, ms_inc);
     }
   };
   for (int b = 0; b < best.tune.nb; ++b) {
