@@ -120,7 +120,7 @@ typedef struct sop_system_desc {
 
 typedef struct sop_engine_opts {
   int32_t nl_cap_max;     /* per-bead neighbor-list capacity bound (default 4096) */
-  int32_t pl_cap_max;     /* per-bead pair-list capacity bound (default 1024, at most 65535) */
+  int32_t pl_cap_max;     /* per-bead pair-list capacity bound (default 1024, at most 4095) */
   int32_t cell_cap_max;   /* per-system cell-grid capacity bound (default 32768) */
   int32_t lanes_per_bead; /* force-kernel lanes per bead: 4, 8, 16 or 32 (default 4; sop_autotune picks one) */
   int32_t grid_blocks;    /* persistent grid size; 0 = 8 per compute unit */

@@ -150,7 +150,7 @@ static void bind_arrays(sop_engine *e) {
       {(void **)&p.cell_pos, 4},   {(void **)&p.nat_sorted, 8},
       {(void **)&p.fout, 16},      {(void **)&p.epart, 8 * EPART_N}, {(void **)&p.vel, 16},
       {(void **)&p.tile_ord, 1},   {(void **)&p.slrep_cnt, 4},   {(void **)&p.pos_shell, 16},
-      {(void **)&p.plrep_off, 4},  {(void **)&p.platt_off, 4}};
+      {(void **)&p.plrep_off, 4},  {(void **)&p.platt_off, 4},  {(void **)&p.plhead, 8}};
   e->nat_arrays = {{(void **)&p.nat_j, 4},    {(void **)&p.nat_r0, 4},  {(void **)&p.nlatt_j, 4},
                    {(void **)&p.nlatt_r0, 4}, {(void **)&p.platt, 8}};
   e->cell_arrays = {{(void **)&p.cell_cnt, 4}, {(void **)&p.cell_start, 4}, {(void **)&p.cell_cursor, 4},
@@ -393,6 +393,13 @@ sop_status engine_add_system_nolists(sop_engine *e, const sop_system_desc *d, in
     for (size_t q = 1; q < nat[i].size(); ++q)
       if (nat[i][q].first == nat[i][q - 1].first) return set_error(e, SOP_ERR_INVALID, "duplicate native pair");
     if (nat[i].size() > 65535) return set_error(e, SOP_ERR_CAPACITY, "more than 65535 native partners of one bead");
+  }
+  // the step kernel holds a native list's offset within its tile's native
+  // slots in 16 bits
+  for (int t0 = 0; t0 < n; t0 += TILE) {
+    size_t tn = 0;
+    for (int i = t0; i < std::min(n, t0 + TILE); ++i) tn += nat[i].size();
+    if (tn > 65535) return set_error(e, SOP_ERR_CAPACITY, "more than 65535 native partners in one 64-bead tile");
   }
   // capacities
   const int32_t nl_cap = round_up(std::max(1, std::min(n - 1, e->opts.nl_cap_max)), 16);
@@ -1294,33 +1301,36 @@ void sop_engine_opts_default(sop_engine_opts *o) {
 sop_status sop_engine_create(int32_t device, const sop_engine_opts *opts, sop_engine **out) {
   if (!out) return SOP_ERR_INVALID;
   *out = nullptr;
+  // the options first (refused before any device call)
+  sop_engine_opts eo;
+  sop_engine_opts_default(&eo);
+  if (opts) {
+    eo = *opts;
+    sop_engine_opts def;
+    sop_engine_opts_default(&def);
+    if (eo.nl_cap_max <= 0) eo.nl_cap_max = def.nl_cap_max;
+    if (eo.pl_cap_max <= 0) eo.pl_cap_max = def.pl_cap_max;
+    if (eo.cell_cap_max <= 0) eo.cell_cap_max = def.cell_cap_max;
+    if (eo.lanes_per_bead <= 0) eo.lanes_per_bead = def.lanes_per_bead;
+    if (eo.lds_beads_max == 0) eo.lds_beads_max = def.lds_beads_max;
+    if (eo.pl_shell == 0.0f) eo.pl_shell = def.pl_shell;
+    if (!(eo.pl_shell > 0.0f) || eo.pl_shell > 8.0f) eo.pl_shell = 0.0f;
+  }
+  const int lpb = eo.lanes_per_bead;
+  const int stt = eo.step_tiles;
+  // the step kernel holds a bead's pair-list and native pair-list lengths in
+  // 16 bits each, and a row's offset within its tile's region in 16 bits
+  // (units of ROW_PAD entries): pl_cap_max <= 4095 (natives per bead and per
+  // tile are checked at add)
+  if ((lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) ||
+      (stt != 0 && stt != 1 && stt != 2 && stt != 4 && stt != 8 && stt != 16) || eo.pl_cap_max > 4095)
+    return SOP_ERR_INVALID;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SOP_ERR_NO_DEVICE;
   if (device < 0 || device >= count) return SOP_ERR_NO_DEVICE;
   (void)hipGetLastError();  // do not inherit an error left by earlier calls
   sop_engine *e = new sop_engine();
-  sop_engine_opts_default(&e->opts);
-  if (opts) {
-    e->opts = *opts;
-    sop_engine_opts def;
-    sop_engine_opts_default(&def);
-    if (e->opts.nl_cap_max <= 0) e->opts.nl_cap_max = def.nl_cap_max;
-    if (e->opts.pl_cap_max <= 0) e->opts.pl_cap_max = def.pl_cap_max;
-    if (e->opts.cell_cap_max <= 0) e->opts.cell_cap_max = def.cell_cap_max;
-    if (e->opts.lanes_per_bead <= 0) e->opts.lanes_per_bead = def.lanes_per_bead;
-    if (e->opts.lds_beads_max == 0) e->opts.lds_beads_max = def.lds_beads_max;
-    if (e->opts.pl_shell == 0.0f) e->opts.pl_shell = def.pl_shell;
-    if (!(e->opts.pl_shell > 0.0f) || e->opts.pl_shell > 8.0f) e->opts.pl_shell = 0.0f;
-  }
-  const int lpb = e->opts.lanes_per_bead;
-  const int stt = e->opts.step_tiles;
-  // the step kernel holds a bead's pair-list and native pair-list lengths in
-  // 16 bits each: pl_cap_max <= 65535 (natives per bead are checked at add)
-  if ((lpb != 4 && lpb != 8 && lpb != 16 && lpb != 32) ||
-      (stt != 0 && stt != 1 && stt != 2 && stt != 4 && stt != 8 && stt != 16) || e->opts.pl_cap_max > 65535) {
-    delete e;
-    return SOP_ERR_INVALID;
-  }
+  e->opts = eo;
   e->device = device;
   if (hipSetDevice(device) != hipSuccess) {
     delete e;

@@ -1216,10 +1216,14 @@ __device__ __forceinline__ void wave_tile(const DevPtrs &p, const SysDev &S, int
   // the host checks)
   int t_ns = 0, t_cn = 0, t_po = 0;
   float t_br = 0.0f;  // r0 of bond (b0 + lane, b0 + lane + 1)
+  // (from the bead's list head: rows from the tile's region start, native
+  // lists from the native slots of the tile's first bead)
+  const int nat0 = p.nat_start[b0], reg0 = (b0 - bead_off) * S.pl_cap;
   if (b0 + lane < bend) {
-    t_ns = p.platt_off[b0 + lane];
-    t_cn = p.platt_cnt[b0 + lane] | (p.plrep_cnt[b0 + lane] << 16);
-    t_po = p.plrep_off[b0 + lane];
+    const uint2 hd = p.plhead[b0 + lane];
+    t_cn = (int)hd.x;
+    t_po = reg0 + ROW_PAD * (int)(hd.y & 0xFFFFu);
+    t_ns = nat0 + (int)(hd.y >> 16);
     t_br = p.bond_r0[b0 + lane];
   }
   const float br_before = b0 > bead_off ? p.bond_r0[b0 - 1] : 0.0f;  // bond (b0 - 1, b0)
@@ -1709,7 +1713,13 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     // the round's starts.  A batch past the first is loaded again for the
     // writes; a bead with more than 128 natives (4 lanes x 32) has its
     // decisions made again.
-    if (!nlb) {
+    int h_ncnt = 0, h_noff = 0;  // the bead's native pair-list length and offset in the tile's slots
+    if (nlb) {
+      if (act) {
+        h_ncnt = p.platt_cnt[i];
+        h_noff = p.platt_off[i] - nat_region;
+      }
+    } else {
       const bool multi = __ballot(nn > 4 * NB) != 0, huge = __ballot(nn > 4 * 32) != 0;
       uint32_t km = 0;  // keep bits: bit m0 + b for the lane's entry l + 4 (m0 + b)
       auto nat_pass = [&](bool wr, int o0) -> int {
@@ -1758,6 +1768,8 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
         p.platt_cnt[i] = ncnt;
         p.platt_off[i] = nstart;
       }
+      h_ncnt = ncnt;
+      h_noff = nstart - nat_region;
     }
     // row starts: the round's rows follow the rows packed so far, in slot order
     const int c = act ? min(cnt, pl_cap) : 0;
@@ -1769,6 +1781,8 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
     if (act && l == 0) {
       p.plrep_cnt[i] = c;
       p.plrep_off[i] = start;
+      p.plhead[i] = make_uint2((uint32_t)h_ncnt | ((uint32_t)c << 16),
+                               (uint32_t)((start - region) / ROW_PAD) | ((uint32_t)h_noff << 16));
       if (cnt > pl_cap) atomicOr(&p.sys[s].err, ERR_PL_OVF);
     }
     // pad to a multiple of ROW_PAD with the bead itself (its d2 = 0 fails the

@@ -124,6 +124,11 @@ struct DevPtrs {
                         // start (pl_base).  Tile k of a system owns entries [64 k pl_cap, 64 (k + 1) pl_cap)
                         // of the segment; its rows lie one after another from the region's start in the
                         // tile's bead order (tile_ord), each padded to a multiple of ROW_PAD entries.
+  uint2 *plhead;        // per bead, what the step kernel reads of the two lists, written by the shell
+                        // filter: {native pair-list length | pair-list length << 16, (plrep_off - the
+                        // tile's region start) / ROW_PAD | (platt_off - nat_start of the tile's first
+                        // bead) << 16}; all four fit 16 bits (pl_cap <= 4095 and natives per tile
+                        // <= 65535, both checked on the host)
   int32_t *slrep_j; int32_t *slrep_cnt;                  // shell rows: row i at (i - bead_off) * pl_cap of the
                                                          // segment, same entry format
   float4 *pos_shell;                                     // coordinates at the last shell-row build

@@ -71,6 +71,24 @@ def test_engine_without_device_fails_loudly():
         S.Engine(0)
 
 
+@pytest.mark.parametrize("field,value", [("pl_cap_max", 4096), ("lanes_per_bead", 2), ("step_tiles", 3)])
+def test_options_out_of_range_are_refused_before_the_device(field, value):
+    # the step kernel holds a row's offset in its tile's region in 16 bits
+    # (units of 4 entries): capacities above 4095 entries are refused
+    o = _abi.EngineOpts()
+    S.lib().sop_engine_opts_default(C.byref(o))
+    setattr(o, field, value)
+    h = C.c_void_p()
+    assert S.lib().sop_engine_create(0, C.byref(o), C.byref(h)) == _abi.SOP_ERR_INVALID
+    o2 = _abi.EngineOpts()
+    S.lib().sop_engine_opts_default(C.byref(o2))
+    o2.pl_cap_max = 4095
+    st = S.lib().sop_engine_create(0, C.byref(o2), C.byref(h))
+    if st == _abi.SOP_OK:
+        S.lib().sop_engine_destroy(h)
+    assert st in (_abi.SOP_OK, _abi.SOP_ERR_NO_DEVICE)
+
+
 def test_parameter_sets():
     r = S.params_reference()
     assert (r.k_bnd, r.R0, r.sigma_ss, r.n_nl, r.n_pl, r.nl_factor, r.pl_factor) == (20, 2, 3.5, 50, 1, 3.2, 2.5)
