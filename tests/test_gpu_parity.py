@@ -370,6 +370,16 @@ def test_malformed_systems_are_rejected(eng):
         with pytest.raises(S.SopError) as ei:
             eng.add(s)
         assert ei.value.status == S._abi.SOP_ERR_INVALID, what
+    # more native partners in one 64-bead tile than the step kernel's 16-bit
+    # native offsets reach: each of beads 0..63 paired with the 1,030 beads
+    # 67..1,096 (65,920 natives in tile 0), refused as a capacity error
+    big = S.make_chain_system(82, 1100)
+    ii, jj = np.meshgrid(np.arange(64, dtype=np.int32), np.arange(67, 1097, dtype=np.int32), indexing="ij")
+    big.nat_i, big.nat_j = ii.ravel(), jj.ravel()
+    big.nat_r0 = np.full(big.nat_i.size, 6.0, np.float32)
+    with pytest.raises(S.SopError) as ei:
+        eng.add(big)
+    assert ei.value.status == S._abi.SOP_ERR_CAPACITY
     s = S.make_chain_system(80, 10)
     s.params.n_nl = 0
     with pytest.raises(S.SopError) as ei:
