@@ -119,8 +119,12 @@ StepGrids step_grids(const sop_engine *e) {
   return StepGrids{grid_items(e), grid_items_big(e), grid_items_far(e), e->cur_lds, e->cur_lds_big};
 }
 
+// The shell filter's LDS launch: one block per item (a block leaves its
+// compute unit after each item, so the cell-grid chain on the other stream
+// finds units between the items); fixed grids (serving) as the step kernel's.
 StepGrids filter_grids(const sop_engine *e) {
-  return StepGrids{items_grid(e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu, e->fixed_grids, e->cur_nfitems),
+  const int g = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
+  return StepGrids{items_grid(e->fixed_grids ? g : std::max(g, 8 * e->num_cu), e->fixed_grids, e->cur_nfitems),
                    items_grid_big(e, e->cur_nfitems_big),
                    items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nfitems_far), e->cur_flds, e->cur_flds_big};
 }
