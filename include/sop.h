@@ -135,8 +135,8 @@ typedef struct sop_engine_opts {
   int32_t step_tiles;     /* 64-bead tiles per work item of the step kernel: 16, 8, 4, 2 or 1;
                              an item of t tiles runs each tile on min(lanes_per_bead, 16 / t)
                              waves (0 = 16 while the batch fills the device, fewer below) */
-  int32_t step_blocks;    /* step-kernel grid in 1024-thread blocks (0 = 2 per compute unit;
-                             sop_autotune picks one) */
+  int32_t step_blocks;    /* step-kernel grid in 1024-thread blocks (0 = 4 per compute unit for
+                             a batch, 2 while serving; sop_autotune picks one) */
   int32_t step_tail;      /* 2 or 4: step items that start after a first round of full items
                              are cut to 16 / step_tail tiles with more waves per tile (a
                              shorter last round); 0 = off (default 2) */

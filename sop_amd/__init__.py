@@ -214,7 +214,7 @@ class Engine:
         if pl_shell is not None:
             o.pl_shell = pl_shell  # < 0: no shell (every pair-list rebuild scans the cell grid)
         o.step_tiles = step_tiles  # tiles per step-kernel item (0: by batch size)
-        o.step_blocks = step_blocks  # step-kernel grid in blocks (0: 2 per compute unit)
+        o.step_blocks = step_blocks  # step-kernel grid in blocks (0: 4 per compute unit, 2 while serving)
         if step_tail is not None:
             o.step_tail = step_tail  # 2 or 4: late step items cut finer (0: off; default 2)
         h = C.c_void_p()

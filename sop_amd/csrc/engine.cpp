@@ -99,8 +99,15 @@ static int items_grid(int g, bool fixed, int count) {
   return std::max(8, std::min(g, (count + 7) / 8 * 8));
 }
 
+// The step kernel's LDS launch: `step_blocks`, by default four blocks per
+// compute unit for a batch (about one block per item of a large batch: two
+// are resident per unit and the dispatcher starts each on whichever unit
+// frees first; A/B on one box, 512 replicas: 6.61e9 -> 6.90e9 bead-steps/s
+// against two per unit) and two while serving (fixed grids).
+int default_step_blocks(const sop_engine *e) { return (e->fixed_grids ? 2 : 4) * e->num_cu; }
+
 int grid_items(const sop_engine *e) {
-  return items_grid(e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu, e->fixed_grids, e->cur_nitems);
+  return items_grid(e->step_blocks > 0 ? e->step_blocks : default_step_blocks(e), e->fixed_grids, e->cur_nitems);
 }
 
 int grid_items_far(const sop_engine *e) { return items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nitems_far); }
@@ -123,7 +130,7 @@ StepGrids step_grids(const sop_engine *e) {
 // compute unit after each item, so the cell-grid chain on the other stream
 // finds units between the items); fixed grids (serving) as the step kernel's.
 StepGrids filter_grids(const sop_engine *e) {
-  const int g = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
+  const int g = e->step_blocks > 0 ? e->step_blocks : default_step_blocks(e);
   return StepGrids{items_grid(e->fixed_grids ? g : std::max(g, 8 * e->num_cu), e->fixed_grids, e->cur_nfitems),
                    items_grid_big(e, e->cur_nfitems_big),
                    items_grid(2 * e->num_cu, e->fixed_grids, e->cur_nfitems_far), e->cur_flds, e->cur_flds_big};

@@ -166,6 +166,7 @@ int grid_sys(const sop_engine *e);
 int grid_items(const sop_engine *e);
 int grid_items_far(const sop_engine *e);
 int grid_items_big(const sop_engine *e);
+int default_step_blocks(const sop_engine *e);
 StepGrids step_grids(const sop_engine *e);
 StepGrids filter_grids(const sop_engine *e);
 ListGrids list_grids(const sop_engine *e);

@@ -142,7 +142,7 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
   const int64_t G = std::max(1, P0.n_nl);
   Setting base;
   base.lanes = e->opts.lanes_per_bead;
-  base.blocks = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
+  base.blocks = e->step_blocks > 0 ? e->step_blocks : default_step_blocks(e);
   base.lds_beads = e->opts.lds_beads_max;
   base.tune = e->tune;
   if (base.tune.nb == 0) {
