@@ -142,7 +142,10 @@ sop_status sop_autotune_ex(sop_engine *e, const int32_t *sys, int32_t nsys, sop_
   const int64_t G = std::max(1, P0.n_nl);
   Setting base;
   base.lanes = e->opts.lanes_per_bead;
-  base.blocks = e->step_blocks > 0 ? e->step_blocks : default_step_blocks(e);
+  // the search starts from two blocks per compute unit, the serving grid:
+  // a served mix tuned on a resident batch keeps it unless more blocks gain
+  // (fixed serving grids ran 1 % below with the batch default of four)
+  base.blocks = e->step_blocks > 0 ? e->step_blocks : 2 * e->num_cu;
   base.lds_beads = e->opts.lds_beads_max;
   base.tune = e->tune;
   if (base.tune.nb == 0) {
