@@ -1588,10 +1588,13 @@ __device__ __forceinline__ void filter_tile(const DevPtrs &p, const SysDev &S, i
   const int t_ns = tin ? p.nat_start[t * TILE + lane] : 0, t_nn = tin ? p.nat_cnt[t * TILE + lane] : 0;
   // the tile's bead order for this filter and the step kernel (k_tile_order's
   // key: shell-row length + the native pair-list length of the last rebuild,
-  // decreasing, ties by index), read before this pass rewrites platt_cnt;
-  // neighbor-list builds keep the order k_tile_order gave them
+  // decreasing, ties by index), read before this pass rewrites platt_cnt.
+  // Computed at grid builds only: shell builds keep the order of the grid
+  // build that made their shell rows (row lengths change little between
+  // them), and neighbor-list builds the order k_tile_order gave them.  The
+  // order only groups beads into rounds; no result depends on it.
   int ordm;
-  if (nlb) {
+  if (nlb || S.build == BUILD_SHELL) {
     ordm = p.tile_ord[t * TILE + lane];
   } else {
     const int key = tin ? t_nr + p.platt_cnt[t * TILE + lane] : -1;
