@@ -80,3 +80,34 @@ def test_trajectory_with_and_without_shell_agree():
         # same lists at every rebuild; only the summation order of the rows differs
         assert np.sqrt(((xa - xb) ** 2).sum(1).mean()) <= 1e-4
         assert a.list_stats(ia).shell_builds > 0
+
+
+def test_batch_mixing_shell_grid_and_neighbor_list_builds():
+    # one batch whose rebuilds mix the three kinds: shell-row filters, grid
+    # builds (large moves) and neighbor-list builds (a 0.05 A skin); the
+    # shell-built systems are filtered beside the cell-grid chain and the
+    # others after it.  Each system ends bitwise as it does alone; the tight
+    # one also matches the oracle's trajectory
+    normal = S.make_chain_system(41, 1500)
+    tight = S.make_chain_system(31, 400)
+    tight.params.nl_abs, tight.params.pl_abs = 16.05, 16.0
+    small = S.make_chain_system(42, 300)
+    small.params.T = 0.9
+    systems = [normal, tight, small]
+    steps = 120
+    with S.Engine(0) as e:
+        ids = [e.add(s) for s in systems]
+        e.run(ids, steps)
+        got = [e.coords(i) for i in ids]
+        st = e.list_stats(ids[1])
+        assert st.nl_builds > 0, st.nl_builds
+        assert e.list_stats(ids[0]).shell_builds > 0
+    with S.Engine(0) as e:
+        for s, x in zip(systems, got):
+            i = e.add(s)
+            e.run([i], steps)
+            assert np.array_equal(e.coords(i), x)
+            e.remove(i)
+    a5, var = S.params_derived(tight.params)
+    xr = Oracle(tight).run(tight.xyz, 0, steps, tight.params.seed, a5, var, tight.params.n_nl, tight.params.n_pl)
+    assert np.sqrt(((got[1] - xr) ** 2).sum(1).mean()) <= 1e-4
